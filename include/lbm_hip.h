@@ -1,0 +1,184 @@
+/*
+ * lbm_hip.h -- C-ABI of the MI355X (gfx950) D2Q9 BGK collide-stream library.
+ *
+ * Plain C: opaque context, plain pointers and sizes, int status returns
+ * (0 = ok, negative = error; lbm_hip_last_error() describes the last one).
+ * No entry point ever calls exit().
+ *
+ * The entry points replace the reference's lattice-core and boundary API
+ * (IUPU-HPC/memory-aware-LBM lb.h, boundaries.h) on one GPU.  The host-side
+ * mirror of that API (include/lb.h, include/boundaries.h) is built on them.
+ *
+ * Host array layout used by upload/download ("node layout"): the reference's
+ * Simulation buffers without the dynamics pointer, i.e. (lx+2)*(ly+2) nodes,
+ * node (x, y) at index x*(ly+2)+y (y contiguous, lb.c:89-100), each node
+ * holding its 9 populations contiguously in the D2Q9 order of lb.c:51-55.
+ */
+#ifndef LBM_HIP_H
+#define LBM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lbm_ctx lbm_ctx;
+
+/* Status codes. */
+enum {
+    LBM_OK = 0,
+    LBM_ERR_ARG = -1,          /* invalid argument or unsupported configuration */
+    LBM_ERR_HIP = -2,          /* a HIP runtime call failed */
+    LBM_ERR_NOMEM = -3,        /* device memory exhausted */
+    LBM_ERR_UNSUPPORTED = -4,  /* dynamics / variant not available on the device */
+    LBM_ERR_STATE = -5         /* call not valid in the context's current state */
+};
+
+/* Sweep algorithms (the reference's five, lb.h:99-122).  On the device:
+ *   ORIGIN      two kernels per step: collide in place, then stream
+ *               (collide + propagate, lb.c:172-341)
+ *   COMBINE     one fused pull kernel per step (collide_with_stream,
+ *               lb.c:353-437)
+ *   TIGHT       the fused kernel advanced two steps per call
+ *               (collide_tight, lb.c:442-931)
+ *   TIGHT_BLOCK two time steps per launch in LDS-resident 2-D tiles
+ *               (collide_tight_block, lb.c:936-1289)
+ *   PANEL       two time steps per launch, each workgroup marching a panel
+ *               with a rolling LDS window (collide_tight_panel_iy,
+ *               lb.c:1402-1779)
+ * All five give bit-identical FP64 results, as the reference's serial ones do. */
+typedef enum {
+    LBM_ORIGIN = 0,
+    LBM_COMBINE = 1,
+    LBM_TIGHT = 2,
+    LBM_TIGHT_BLOCK = 3,
+    LBM_PANEL = 4
+} lbm_variant;
+
+/* Population storage.  F64 is the reference's arithmetic.  F32/F16 store the
+ * deviations f_i - w_i and collide in FP32, keeping FP64 for the wall
+ * operators' moments and for reductions (no reference counterpart). */
+typedef enum { LBM_STORE_F64 = 0, LBM_STORE_F32 = 1, LBM_STORE_F16 = 2 } lbm_storage;
+
+/* Straight-wall operator families and kinds (boundaries.h:72-102). */
+typedef enum { LBM_BC_REGULARIZED = 0, LBM_BC_ZOUHE = 1 } lbm_bc_family;
+typedef enum { LBM_BC_VELOCITY = 0, LBM_BC_PRESSURE = 1 } lbm_bc_kind;
+typedef enum { LBM_SIDE_UPPER = 0, LBM_SIDE_LOWER = 1, LBM_SIDE_LEFT = 2,
+               LBM_SIDE_RIGHT = 3 } lbm_side;
+
+/* One side of the channel: every node of row 1 (lower), row ly (upper),
+ * column 1 rows 2..ly-1 (left) or column lx rows 2..ly-1 (right) carries
+ * this operator (unsteady.c:185-198 assign them in that precedence).
+ * p0/p1 are per-position parameters -- (ux, uy) for a velocity operator,
+ * (rho, uPar) for a pressure operator (VelocityBCData / PressureBCData,
+ * boundaries.h:50-67) -- indexed by x (0..lx+1) on upper/lower sides and by
+ * y (0..ly+1) on left/right sides.  NULL means "all zero" (p0) / "all zero"
+ * (p1); for a pressure side a NULL p0 means rho = 1. */
+typedef struct {
+    int kind;          /* lbm_bc_kind */
+    int family;        /* lbm_bc_family */
+    const double *p0;
+    const double *p1;
+} lbm_side_bc;
+
+/* The cylinder channel of the reference driver (unsteady.c:84-199): bulk
+ * BGK with relaxation omega, bounce-back on the integer disc
+ * (x-obst_x)^2 + (y-obst_y)^2 <= obst_r^2, and one straight-wall operator per
+ * side.  With zgb != 0 the right side's density is re-extrapolated every step
+ * from columns lx-1 and lx-2 (updateZeroGradientBoundary, unsteady.c:205-214)
+ * on the device; the right side must then be a pressure side. */
+typedef struct {
+    int lx, ly;
+    int obst_x, obst_y, obst_r;   /* obst_r < 0: no disc */
+    double omega;
+    lbm_side_bc side[4];          /* indexed by lbm_side */
+    int zgb;
+    int storage;                  /* lbm_storage */
+} lbm_problem;
+
+/* Create a device context for a problem on the current HIP device; allocates
+ * the two population buffers (2 * 9 * (lx+2) * pitch elements) and the
+ * per-side parameter arrays.  Replaces constructSim + setDynamics
+ * (lb.c:85-121) for the device.  The populations start at zero, as calloc
+ * leaves the reference's buffers. */
+int lbm_hip_create(const lbm_problem *prob, lbm_ctx **out);
+int lbm_hip_destroy(lbm_ctx *ctx);                               /* destructSim, lb.c:109-115 */
+
+/* Device-side initial condition of the reference driver (iniGeometry,
+ * unsteady.c:162-199 with iniEquilibrium, lb.c:71-78): every interior node of
+ * the lattice set to equilibrium at rho = 1, u = (init_ux[y], 0), the tmp
+ * buffer zeroed, the step counter reset. init_ux has ly+2 entries. */
+int lbm_hip_init_equilibrium(lbm_ctx *ctx, const double *init_ux);
+
+/* Copy a reference-convention state (Simulation.lattice = post-stream
+ * populations, Simulation.tmpLattice = what the last sweep left there) in
+ * node layout to / from the device.  Only interior nodes are transferred;
+ * download leaves the ghost ring of the host arrays untouched.  outlet_rho
+ * (ly+2 entries, may be NULL) is pressureBoundary[y].rho. */
+int lbm_hip_upload(lbm_ctx *ctx, const double *lattice, const double *tmp,
+                   const double *outlet_rho);
+int lbm_hip_download(lbm_ctx *ctx, double *lattice, double *tmp, double *outlet_rho);
+
+/* Advance nsteps time steps with a sweep variant (lb.h:99-122, driver loop
+ * unsteady.c:425-531 including the zero-gradient outlet update).  Enqueued on
+ * the context's stream; returns without waiting. */
+int lbm_hip_step(lbm_ctx *ctx, int variant, int nsteps);
+
+/* Block-size knobs of the blocked variants (the reference's blk_size /
+ * thread_block, unsteady.c:93,258): tile extent in x and y.  0 = default. */
+int lbm_hip_set_tile(lbm_ctx *ctx, int tile_x, int tile_y);
+
+/* Replace the per-position parameters of one side (e.g. a new inlet profile
+ * or, with zgb off, prescribed outlet densities). */
+int lbm_hip_set_side_params(lbm_ctx *ctx, int side, const double *p0, const double *p1);
+
+/* Macroscopic fields of the current post-stream state (computeMacros,
+ * lb.c:134-141, on every interior node): arrays of lx*ly doubles, x slowest
+ * (the order of lattice[iX][iY]).  Any pointer may be NULL. */
+int lbm_hip_macros(lbm_ctx *ctx, double *rho, double *ux, double *uy);
+
+/* Diagnostics of the current post-stream state, reduced on the device in a
+ * fixed order: out[0] drag and out[1] lift on the disc by momentum exchange,
+ * out[2] kinetic energy sum(rho |u|^2 / 2) and out[3] mass over non-solid
+ * nodes.  No counterpart in the reference. */
+int lbm_hip_diagnostics(lbm_ctx *ctx, double out[4]);
+
+/* Steps taken since init/upload, and the state bookkeeping. */
+int64_t lbm_hip_steps_done(const lbm_ctx *ctx);
+
+/* Wait for all work of the context. */
+int lbm_hip_sync(lbm_ctx *ctx);
+
+/* Time nsteps of a variant with HIP events on the context's stream (work is
+ * enqueued exactly as lbm_hip_step does).  ms_total receives the elapsed
+ * milliseconds; kernel_ms (may be NULL) receives the average duration of one
+ * launch of the variant's dominant kernel, measured with events around a
+ * separate batch of launches of that kernel alone. */
+int lbm_hip_time_steps(lbm_ctx *ctx, int variant, int nsteps, double *ms_total);
+
+/* Device bytes one step of a variant moves by design (populations read plus
+ * written), per interior node. */
+double lbm_hip_bytes_per_node(const lbm_ctx *ctx, int variant);
+
+/* Total device memory the context holds, in bytes. */
+size_t lbm_hip_footprint(const lbm_ctx *ctx);
+/* Device memory a problem would need, without allocating. */
+size_t lbm_hip_footprint_of(const lbm_problem *prob);
+
+/* Known-answer entry: apply one node operator of the device to n population
+ * vectors (9 doubles each, in place, host memory).  which follows the order
+ * of boundaries.h:74-102 (0 bounceBack, 1..4 Zou/He velocity upper, lower,
+ * left, right, 5..8 Zou/He pressure, 9..12 regularized velocity, 13..16
+ * regularized pressure), plus 17 = bgk (lb.c:154-166).  p0/p1 per vector. */
+int lbm_hip_apply_node_op(int which, int n, double *f, const double *p0,
+                          const double *p1, double omega);
+
+/* Description of the last error on this thread. */
+const char *lbm_hip_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LBM_HIP_H */

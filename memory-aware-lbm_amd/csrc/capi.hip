@@ -1,0 +1,452 @@
+// capi.hip -- the extern "C" surface of include/lbm_hip.h: device contexts,
+// state bookkeeping between the two state conventions, host transfers and
+// the variant dispatch.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/lbm_hip.h"
+#include "launch.hpp"
+
+using namespace lbm;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+    std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    return fail(e == hipErrorOutOfMemory ? LBM_ERR_NOMEM : LBM_ERR_HIP, m);
+}
+
+#define HIPCHK(expr)                                    \
+    do {                                                \
+        hipError_t e_ = (expr);                         \
+        if (e_ != hipSuccess) return hip_fail(e_, #expr); \
+    } while (0)
+
+size_t elem_size(int storage) { return storage == LBM_STORE_F64 ? 8 : storage == LBM_STORE_F32 ? 4 : 2; }
+
+}  // namespace
+
+struct lbm_ctx {
+    Problem prob;
+    int storage;
+    void *buf[2];
+    int cur;         // buffer holding L (streamed) or P (collided)
+    int collided;    // state convention of buf[cur]
+    int64_t steps;
+    double *d_p0[4], *d_p1[4];
+    double *d_outlet;
+    double *d_partial;
+    double *d_out4;
+    int64_t npartial;
+    hipStream_t stream;
+    int tile_x, tile_y;
+    size_t footprint;
+};
+
+namespace {
+
+int side_len(const lbm_problem *p, int s) {
+    return (s == LBM_SIDE_UPPER || s == LBM_SIDE_LOWER) ? p->lx + 2 : p->ly + 2;
+}
+
+int validate(const lbm_problem *p) {
+    if (!p) return fail(LBM_ERR_ARG, "null problem");
+    if (p->lx < 3 || p->ly < 3) return fail(LBM_ERR_ARG, "lx and ly must be >= 3");
+    if ((int64_t)(p->lx + 2) * (p->ly + 2) > (int64_t)1 << 31)
+        return fail(LBM_ERR_ARG, "lattice too large for 32-bit row indexing");
+    if (!(p->omega > 0.0 && p->omega < 2.0))
+        return fail(LBM_ERR_ARG, "omega must lie in (0, 2)");
+    for (int s = 0; s < 4; ++s) {
+        const lbm_side_bc &b = p->side[s];
+        if (b.kind != LBM_BC_VELOCITY && b.kind != LBM_BC_PRESSURE)
+            return fail(LBM_ERR_ARG, "side kind must be velocity or pressure");
+        if (b.family != LBM_BC_REGULARIZED && b.family != LBM_BC_ZOUHE)
+            return fail(LBM_ERR_ARG, "side family must be regularized or Zou/He");
+    }
+    if (p->zgb && p->side[LBM_SIDE_RIGHT].kind != LBM_BC_PRESSURE)
+        return fail(LBM_ERR_ARG, "zgb needs a pressure operator on the right side");
+    if (p->storage != LBM_STORE_F64)
+        return fail(LBM_ERR_UNSUPPORTED, "storage format not available in this build");
+    return LBM_OK;
+}
+
+int upload_side(lbm_ctx *c, int s, const lbm_problem *p, const double *p0, const double *p1) {
+    int n = side_len(p, s);
+    std::vector<double> a(n, 0.0), b(n, 0.0);
+    if (p0) std::memcpy(a.data(), p0, n * sizeof(double));
+    else if (p->side[s].kind == LBM_BC_PRESSURE) std::fill(a.begin(), a.end(), 1.0);
+    if (p1) std::memcpy(b.data(), p1, n * sizeof(double));
+    HIPCHK(hipMemcpy(c->d_p0[s], a.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_p1[s], b.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    if (s == LBM_SIDE_RIGHT)
+        HIPCHK(hipMemcpy(c->d_outlet, a.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    return LBM_OK;
+}
+
+double *dbuf(lbm_ctx *c, int k) { return static_cast<double *>(c->buf[k]); }
+
+// Bring the context into the collided convention (P = buf[cur]).
+int to_collided(lbm_ctx *c) {
+    if (c->collided) return LBM_OK;
+    // one collision of step m from the streamed state: L -> T
+    HIPCHK(launch_collide_f64(c->prob, false, dbuf(c, c->cur), dbuf(c, 1 - c->cur), c->stream));
+    c->cur = 1 - c->cur;
+    c->collided = 1;
+    c->steps += 1;  // the collided state after this represents step m+1
+    return LBM_OK;
+}
+
+// Bring the context into the streamed convention (L = buf[cur]).
+int to_streamed(lbm_ctx *c) {
+    if (!c->collided) return LBM_OK;
+    HIPCHK(launch_stream_f64(c->prob.lay, dbuf(c, c->cur), dbuf(c, 1 - c->cur), c->stream));
+    c->cur = 1 - c->cur;
+    c->collided = 0;
+    return LBM_OK;
+}
+
+int step_impl(lbm_ctx *c, int variant, int nsteps) {
+    if (nsteps < 0) return fail(LBM_ERR_ARG, "nsteps < 0");
+    if (nsteps == 0) return LBM_OK;
+    switch (variant) {
+    case LBM_ORIGIN: {
+        // collide (lb.c:172-201) then stream (lb.c:246-287), every step
+        int rc = to_streamed(c);
+        if (rc) return rc;
+        for (int k = 0; k < nsteps; ++k) {
+            HIPCHK(launch_collide_f64(c->prob, false, dbuf(c, c->cur), dbuf(c, 1 - c->cur),
+                                      c->stream));
+            HIPCHK(launch_stream_f64(c->prob.lay, dbuf(c, 1 - c->cur), dbuf(c, c->cur),
+                                     c->stream));
+            c->steps += 1;
+        }
+        return LBM_OK;
+    }
+    case LBM_COMBINE:
+    case LBM_TIGHT: {
+        int k = 0;
+        if (!c->collided) {
+            int rc = to_collided(c);
+            if (rc) return rc;
+            k = 1;
+        }
+        for (; k < nsteps; ++k) {
+            HIPCHK(launch_collide_f64(c->prob, true, dbuf(c, c->cur), dbuf(c, 1 - c->cur),
+                                      c->stream));
+            c->cur = 1 - c->cur;
+            c->steps += 1;
+        }
+        return LBM_OK;
+    }
+    case LBM_TIGHT_BLOCK:
+    case LBM_PANEL:
+        return fail(LBM_ERR_UNSUPPORTED, "blocked two-step variants not built yet");
+    default:
+        return fail(LBM_ERR_ARG, "unknown variant");
+    }
+}
+
+// host node layout <-> device planes, interior nodes only
+int transfer(lbm_ctx *c, double *host, int k, bool to_device) {
+    const Layout &L = c->prob.lay;
+    size_t plane_bytes = (size_t)L.plane * sizeof(double);
+    double *stage = nullptr;
+    HIPCHK(hipHostMalloc((void **)&stage, plane_bytes, hipHostMallocDefault));
+    int ny = L.ly + 2;
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < 9 && e == hipSuccess; ++i) {
+        double *dev = dbuf(c, k) + (int64_t)i * L.plane;
+        if (to_device) {
+            e = hipMemcpy(stage, dev, plane_bytes, hipMemcpyDeviceToHost);  // keep ghosts
+            if (e != hipSuccess) break;
+            for (int x = 1; x <= L.lx; ++x)
+                for (int y = 1; y <= L.ly; ++y)
+                    stage[L.node(x, y)] = host[((size_t)x * ny + y) * 9 + i];
+            e = hipMemcpy(dev, stage, plane_bytes, hipMemcpyHostToDevice);
+        } else {
+            e = hipMemcpy(stage, dev, plane_bytes, hipMemcpyDeviceToHost);
+            if (e != hipSuccess) break;
+            for (int x = 1; x <= L.lx; ++x)
+                for (int y = 1; y <= L.ly; ++y)
+                    host[((size_t)x * ny + y) * 9 + i] = stage[L.node(x, y)];
+        }
+    }
+    hipError_t e2 = hipHostFree(stage);
+    if (e != hipSuccess) return hip_fail(e, "population transfer");
+    if (e2 != hipSuccess) return hip_fail(e2, "hipHostFree");
+    return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *lbm_hip_last_error(void) { return g_err.c_str(); }
+
+size_t lbm_hip_footprint_of(const lbm_problem *p) {
+    if (!p || p->lx < 1 || p->ly < 1) return 0;
+    Layout L = make_layout(p->lx, p->ly);
+    size_t pops = 2 * 9 * (size_t)L.plane * elem_size(p->storage);
+    size_t sides = 2 * sizeof(double) * (2 * (size_t)(p->lx + 2) + 2 * (size_t)(p->ly + 2));
+    size_t diag = 4 * sizeof(double) * (size_t)diag_partials_needed(L) + 64;
+    return pops + sides + (size_t)(p->ly + 2) * sizeof(double) + diag;
+}
+
+int lbm_hip_create(const lbm_problem *p, lbm_ctx **out) {
+    if (!out) return fail(LBM_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    int rc = validate(p);
+    if (rc) return rc;
+    lbm_ctx *c = new (std::nothrow) lbm_ctx();
+    if (!c) return fail(LBM_ERR_NOMEM, "host allocation");
+    c->storage = p->storage;
+    c->prob.lay = make_layout(p->lx, p->ly);
+    c->prob.ox = p->obst_x;
+    c->prob.oy = p->obst_y;
+    c->prob.r2 = p->obst_r >= 0 ? p->obst_r * p->obst_r : -1;
+    c->prob.omega = p->omega;
+    c->prob.zgb = p->zgb;
+    c->npartial = diag_partials_needed(c->prob.lay);
+    size_t bytes = 9 * (size_t)c->prob.lay.plane * elem_size(p->storage);
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete c; return hip_fail(e, "hipStreamCreate"); }
+    auto cleanup = [&](hipError_t err, const char *what) {
+        lbm_hip_destroy(c);
+        return hip_fail(err, what);
+    };
+    for (int k = 0; k < 2; ++k) {
+        e = hipMalloc(&c->buf[k], bytes);
+        if (e != hipSuccess) return cleanup(e, "hipMalloc populations");
+        e = hipMemset(c->buf[k], 0, bytes);
+        if (e != hipSuccess) return cleanup(e, "hipMemset populations");
+    }
+    for (int s = 0; s < 4; ++s) {
+        int n = side_len(p, s);
+        e = hipMalloc((void **)&c->d_p0[s], n * sizeof(double));
+        if (e != hipSuccess) return cleanup(e, "hipMalloc side");
+        e = hipMalloc((void **)&c->d_p1[s], n * sizeof(double));
+        if (e != hipSuccess) return cleanup(e, "hipMalloc side");
+    }
+    e = hipMalloc((void **)&c->d_outlet, (p->ly + 2) * sizeof(double));
+    if (e != hipSuccess) return cleanup(e, "hipMalloc outlet");
+    e = hipMalloc((void **)&c->d_partial, 4 * sizeof(double) * c->npartial);
+    if (e != hipSuccess) return cleanup(e, "hipMalloc partials");
+    e = hipMalloc((void **)&c->d_out4, 4 * sizeof(double));
+    if (e != hipSuccess) return cleanup(e, "hipMalloc out");
+    for (int s = 0; s < 4; ++s) {
+        rc = upload_side(c, s, p, p->side[s].p0, p->side[s].p1);
+        if (rc) { std::string m = g_err; lbm_hip_destroy(c); return fail(rc, m); }
+        c->prob.side[s].kind = p->side[s].kind;
+        c->prob.side[s].family = p->side[s].family;
+        c->prob.side[s].p0 = c->d_p0[s];
+        c->prob.side[s].p1 = c->d_p1[s];
+    }
+    c->prob.outlet_rho = c->d_outlet;
+    c->cur = 0;
+    c->collided = 0;
+    c->steps = 0;
+    c->footprint = lbm_hip_footprint_of(p);
+    *out = c;
+    return LBM_OK;
+}
+
+int lbm_hip_destroy(lbm_ctx *c) {
+    if (!c) return LBM_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (int k = 0; k < 2; ++k) if (c->buf[k]) (void)hipFree(c->buf[k]);
+    for (int s = 0; s < 4; ++s) {
+        if (c->d_p0[s]) (void)hipFree(c->d_p0[s]);
+        if (c->d_p1[s]) (void)hipFree(c->d_p1[s]);
+    }
+    if (c->d_outlet) (void)hipFree(c->d_outlet);
+    if (c->d_partial) (void)hipFree(c->d_partial);
+    if (c->d_out4) (void)hipFree(c->d_out4);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return LBM_OK;
+}
+
+int lbm_hip_init_equilibrium(lbm_ctx *c, const double *init_ux) {
+    if (!c || !init_ux) return fail(LBM_ERR_ARG, "null argument");
+    const Layout &L = c->prob.lay;
+    size_t bytes = 9 * (size_t)L.plane * elem_size(c->storage);
+    double *d_ux = nullptr;
+    HIPCHK(hipMalloc((void **)&d_ux, (L.ly + 2) * sizeof(double)));
+    hipError_t e = hipMemcpy(d_ux, init_ux, (L.ly + 2) * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemsetAsync(c->buf[0], 0, bytes, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->buf[1], 0, bytes, c->stream);
+    if (e == hipSuccess) e = launch_init_f64(L, d_ux, dbuf(c, 0), c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d_ux);
+    if (e != hipSuccess) return hip_fail(e, "init_equilibrium");
+    c->cur = 0;
+    c->collided = 0;
+    c->steps = 0;
+    return LBM_OK;
+}
+
+int lbm_hip_upload(lbm_ctx *c, const double *lattice, const double *tmp,
+                   const double *outlet_rho) {
+    if (!c || !lattice || !tmp) return fail(LBM_ERR_ARG, "null argument");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    int rc = transfer(c, const_cast<double *>(lattice), 0, true);
+    if (!rc) rc = transfer(c, const_cast<double *>(tmp), 1, true);
+    if (rc) return rc;
+    if (outlet_rho) {
+        HIPCHK(hipMemcpy(c->d_outlet, outlet_rho, (c->prob.lay.ly + 2) * sizeof(double),
+                         hipMemcpyHostToDevice));
+        if (!c->prob.zgb)
+            HIPCHK(hipMemcpy(c->d_p0[LBM_SIDE_RIGHT], outlet_rho,
+                             (c->prob.lay.ly + 2) * sizeof(double), hipMemcpyHostToDevice));
+    }
+    c->cur = 0;
+    c->collided = 0;
+    c->steps = 0;
+    return LBM_OK;
+}
+
+int lbm_hip_download(lbm_ctx *c, double *lattice, double *tmp, double *outlet_rho) {
+    if (!c) return fail(LBM_ERR_ARG, "null context");
+    int lat_buf = c->cur, tmp_buf = 1 - c->cur;
+    if (c->collided) {
+        // post-stream state into the free buffer; P itself is left as it is
+        HIPCHK(launch_stream_f64(c->prob.lay, dbuf(c, c->cur), dbuf(c, 1 - c->cur), c->stream));
+        lat_buf = 1 - c->cur;
+        tmp_buf = c->cur;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    int rc = LBM_OK;
+    if (lattice) rc = transfer(c, lattice, lat_buf, false);
+    if (!rc && tmp) rc = transfer(c, tmp, tmp_buf, false);
+    if (rc) return rc;
+    if (outlet_rho)
+        HIPCHK(hipMemcpy(outlet_rho, c->d_outlet, (c->prob.lay.ly + 2) * sizeof(double),
+                         hipMemcpyDeviceToHost));
+    return LBM_OK;
+}
+
+int lbm_hip_step(lbm_ctx *c, int variant, int nsteps) {
+    if (!c) return fail(LBM_ERR_ARG, "null context");
+    return step_impl(c, variant, nsteps);
+}
+
+int lbm_hip_set_tile(lbm_ctx *c, int tile_x, int tile_y) {
+    if (!c || tile_x < 0 || tile_y < 0) return fail(LBM_ERR_ARG, "bad tile");
+    c->tile_x = tile_x;
+    c->tile_y = tile_y;
+    return LBM_OK;
+}
+
+int lbm_hip_set_side_params(lbm_ctx *c, int side, const double *p0, const double *p1) {
+    if (!c || side < 0 || side > 3) return fail(LBM_ERR_ARG, "bad side");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    int n = (side == LBM_SIDE_UPPER || side == LBM_SIDE_LOWER) ? c->prob.lay.lx + 2
+                                                               : c->prob.lay.ly + 2;
+    if (p0) HIPCHK(hipMemcpy(c->d_p0[side], p0, n * sizeof(double), hipMemcpyHostToDevice));
+    if (p1) HIPCHK(hipMemcpy(c->d_p1[side], p1, n * sizeof(double), hipMemcpyHostToDevice));
+    return LBM_OK;
+}
+
+int lbm_hip_macros(lbm_ctx *c, double *rho, double *ux, double *uy) {
+    if (!c) return fail(LBM_ERR_ARG, "null context");
+    const Layout &L = c->prob.lay;
+    size_t n = (size_t)L.lx * L.ly;
+    double *d = nullptr;
+    HIPCHK(hipMalloc((void **)&d, 3 * n * sizeof(double)));
+    hipError_t e = launch_macros_f64(L, c->collided, dbuf(c, c->cur), d, d + n, d + 2 * n,
+                                     c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess && rho) e = hipMemcpy(rho, d, n * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && ux) e = hipMemcpy(ux, d + n, n * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && uy)
+        e = hipMemcpy(uy, d + 2 * n, n * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_fail(e, "macros");
+    return LBM_OK;
+}
+
+int lbm_hip_diagnostics(lbm_ctx *c, double out[4]) {
+    if (!c || !out) return fail(LBM_ERR_ARG, "null argument");
+    HIPCHK(launch_diag_f64(c->prob, c->collided, dbuf(c, c->cur), c->d_partial, c->d_out4,
+                           c->stream));
+    HIPCHK(hipMemcpyAsync(out, c->d_out4, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+int64_t lbm_hip_steps_done(const lbm_ctx *c) { return c ? c->steps : -1; }
+
+int lbm_hip_sync(lbm_ctx *c) {
+    if (!c) return fail(LBM_ERR_ARG, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+int lbm_hip_time_steps(lbm_ctx *c, int variant, int nsteps, double *ms_total) {
+    if (!c || !ms_total) return fail(LBM_ERR_ARG, "null argument");
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    // settle the state convention outside the timed region
+    int rc = LBM_OK;
+    if (variant == LBM_ORIGIN) rc = to_streamed(c);
+    hipError_t e = hipEventRecord(a, c->stream);
+    if (!rc && e == hipSuccess) rc = step_impl(c, variant, nsteps);
+    if (!rc && e == hipSuccess) e = hipEventRecord(b, c->stream);
+    if (!rc && e == hipSuccess) e = hipEventSynchronize(b);
+    float ms = 0.f;
+    if (!rc && e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (rc) return rc;
+    if (e != hipSuccess) return hip_fail(e, "event timing");
+    *ms_total = ms;
+    return LBM_OK;
+}
+
+double lbm_hip_bytes_per_node(const lbm_ctx *c, int variant) {
+    if (!c) return 0.0;
+    double pop = 9.0 * elem_size(c->storage);
+    switch (variant) {
+    case LBM_ORIGIN: return 4 * pop;        // collide r+w, stream r+w
+    case LBM_COMBINE:
+    case LBM_TIGHT: return 2 * pop;         // one read and one write per step
+    case LBM_TIGHT_BLOCK:
+    case LBM_PANEL: return pop;             // one read and one write per two steps
+    default: return 0.0;
+    }
+}
+
+size_t lbm_hip_footprint(const lbm_ctx *c) { return c ? c->footprint : 0; }
+
+int lbm_hip_apply_node_op(int which, int n, double *f, const double *p0, const double *p1,
+                          double omega) {
+    if (which < 0 || which > 17 || n < 0 || (n > 0 && (!f || !p0 || !p1)))
+        return fail(LBM_ERR_ARG, "bad node-operator call");
+    if (n == 0) return LBM_OK;
+    double *d = nullptr;
+    HIPCHK(hipMalloc((void **)&d, (size_t)n * 11 * sizeof(double)));
+    hipError_t e = hipMemcpy(d, f, (size_t)n * 9 * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + 9 * (size_t)n, p0, n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + 10 * (size_t)n, p1, n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_node_op_f64(which, n, d, d + 9 * (size_t)n, d + 10 * (size_t)n, omega, 0);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(f, d, (size_t)n * 9 * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_fail(e, "node operator");
+    return LBM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,230 @@
+// d2q9_fp64.hpp -- double-precision D2Q9 node operators for gfx950.
+//
+// These are the per-node numerics of the collide-stream path: moments,
+// equilibrium, BGK relaxation, full-way bounce-back and the straight-wall
+// Zou/He and regularized operators.  Every expression keeps the evaluation
+// order of the reference C code it replaces (cited per function) and this
+// header is only compiled with -ffp-contract=off, so each double operation
+// rounds exactly once, as in the reference's SSE2 build.  That is what makes
+// the device results bit-identical to the reference.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace lbm {
+
+// D2Q9 lattice (lb.c:48-55): 0 rest, 1 E, 2 N, 3 W, 4 S, 5 NE, 6 NW, 7 SW, 8 SE.
+__host__ __device__ constexpr int cx(int i) {
+    return (i == 1 || i == 5 || i == 8) ? 1 : (i == 3 || i == 6 || i == 7) ? -1 : 0;
+}
+__host__ __device__ constexpr int cy(int i) {
+    return (i == 2 || i == 5 || i == 6) ? 1 : (i == 4 || i == 7 || i == 8) ? -1 : 0;
+}
+__host__ __device__ constexpr double wgt(int i) {
+    return i == 0 ? 4. / 9. : (i <= 4 ? 1. / 9. : 1. / 36.);
+}
+__host__ __device__ constexpr int opp(int i) {  // boundaries.c:32
+    return i == 0 ? 0 : (i <= 4 ? ((i + 1) & 3) + 1 : ((i - 3) & 3) + 5);
+}
+
+enum Side : int { UPPER = 0, LOWER = 1, LEFT = 2, RIGHT = 3 };
+
+// Moments (lb.c:134-141): density summed by lines of constant c_y; the
+// velocity by true IEEE division.
+__device__ __forceinline__ double rho_of(const double f[9]) {
+    double north = f[2] + f[5] + f[6];
+    double mid = f[0] + f[1] + f[3];
+    double south = f[4] + f[7] + f[8];
+    return north + mid + south;
+}
+
+__device__ __forceinline__ void macros(const double f[9], double &rho, double &ux,
+                                       double &uy) {
+    double north = f[2] + f[5] + f[6];
+    double mid = f[0] + f[1] + f[3];
+    double south = f[4] + f[7] + f[8];
+    rho = north + mid + south;
+    ux = (f[1] + f[5] + f[8] - (f[3] + f[6] + f[7])) / rho;
+    uy = (north - south) / rho;
+}
+
+// Equilibrium (lb.c:144-151).  c.u is formed from integer components; the
+// products by +-1 and 0 are exact, so the specialised sums below round the
+// same way as cx*ux + cy*uy.
+template <int I>
+__device__ __forceinline__ double cdotu(double ux, double uy) {
+    if constexpr (I == 0) return 0.;
+    else if constexpr (I == 1) return ux;
+    else if constexpr (I == 2) return uy;
+    else if constexpr (I == 3) return -ux;
+    else if constexpr (I == 4) return -uy;
+    else if constexpr (I == 5) return ux + uy;
+    else if constexpr (I == 6) return -ux + uy;
+    else if constexpr (I == 7) return -ux + -uy;
+    else return ux + -uy;
+}
+
+template <int I>
+__device__ __forceinline__ double feq(double rho, double ux, double uy, double usqr) {
+    double cu = cdotu<I>(ux, uy);
+    return rho * wgt(I) * (1. + 3. * cu + 4.5 * cu * cu - 1.5 * usqr);
+}
+
+template <int... Is> struct seq {};
+template <typename F, int... Is>
+__device__ __forceinline__ void for_pops_impl(F &&fn, seq<Is...>) {
+    (fn.template operator()<Is>(), ...);
+}
+template <typename F> __device__ __forceinline__ void for_pops(F &&fn) {
+    for_pops_impl(fn, seq<0, 1, 2, 3, 4, 5, 6, 7, 8>{});
+}
+
+// BGK relaxation (lb.c:154-166): f_i = f_i*(1-omega), then += omega*feq_i.
+__device__ __forceinline__ void bgk(double f[9], double omega) {
+    double rho, ux, uy;
+    macros(f, rho, ux, uy);
+    double usqr = ux * ux + uy * uy;
+    double keep = 1 - omega;
+    for_pops([&]<int I>() {
+        f[I] *= keep;
+        f[I] += omega * feq<I>(rho, ux, uy, usqr);
+    });
+}
+
+// Full-way bounce-back (boundaries.c:48-59).
+__device__ __forceinline__ void bounce_back(double f[9]) {
+    double t;
+    t = f[1]; f[1] = f[3]; f[3] = t;
+    t = f[2]; f[2] = f[4]; f[4] = t;
+    t = f[5]; f[5] = f[7]; f[7] = t;
+    t = f[6]; f[6] = f[8]; f[8] = t;
+}
+
+// Known-population sum at a wall node (boundaries.c:66-126 bracket it as
+// tangential triple plus twice the outgoing triple).
+__device__ __forceinline__ double wall_sum(const double f[9], int s) {
+    switch (s) {
+    case UPPER: return f[0] + f[3] + f[1] + 2 * (f[6] + f[2] + f[5]);
+    case LOWER: return f[0] + f[3] + f[1] + 2 * (f[8] + f[4] + f[7]);
+    case RIGHT: return f[0] + f[2] + f[4] + 2 * (f[1] + f[5] + f[8]);
+    default:    return f[0] + f[2] + f[4] + 2 * (f[3] + f[6] + f[7]);
+    }
+}
+
+// Density from a prescribed normal velocity (upperRho, lowerRho, rightRho,
+// leftRho: boundaries.c:66-118): reciprocal first, then the product.
+__device__ __forceinline__ double wall_rho(const double f[9], int s, double un) {
+    double sgn_den = (s == UPPER || s == RIGHT) ? 1. + un : 1. - un;
+    return 1. / sgn_den * wall_sum(f, s);
+}
+
+// Normal velocity from a prescribed density (upperU, lowerU, rightU, leftU:
+// boundaries.c:74-126).
+__device__ __forceinline__ double wall_un(const double f[9], int s, double rho) {
+    double q = 1. / rho * wall_sum(f, s);
+    return (s == UPPER || s == RIGHT) ? -1. + q : 1. - q;
+}
+
+// Zou/He completion of the three unknown populations (boundaries.c:131-169).
+__device__ __forceinline__ void zh_complete(double f[9], int s, double ux, double uy,
+                                            double rho) {
+    switch (s) {
+    case UPPER:
+        f[7] = f[5] + 0.5 * (f[1] - f[3]) - rho * uy / 6 - rho * ux / 2.;
+        f[8] = f[6] + 0.5 * (f[3] - f[1]) - rho * uy / 6 + rho * ux / 2.;
+        f[4] = f[2] - 2. / 3. * rho * uy;
+        break;
+    case LOWER:
+        f[6] = f[8] + 0.5 * (f[1] - f[3]) + rho * uy / 6 - rho * ux / 2.;
+        f[5] = f[7] + 0.5 * (f[3] - f[1]) + rho * uy / 6 + rho * ux / 2.;
+        f[2] = f[4] + 2. / 3. * rho * uy;
+        break;
+    case RIGHT:
+        f[6] = f[8] + 0.5 * (f[4] - f[2]) - rho * ux / 6 + rho * uy / 2.;
+        f[7] = f[5] + 0.5 * (f[2] - f[4]) - rho * ux / 6 - rho * uy / 2.;
+        f[3] = f[1] - 2. / 3. * rho * ux;
+        break;
+    default:
+        f[5] = f[7] + 0.5 * (f[4] - f[2]) + rho * ux / 6. + rho * uy / 2.;
+        f[8] = f[6] + 0.5 * (f[2] - f[4]) + rho * ux / 6. - rho * uy / 2.;
+        f[1] = f[3] + 2. / 3. * rho * ux;
+        break;
+    }
+}
+
+// Non-equilibrium momentum flux from the known populations (xNeqPi,
+// boundaries.c:266-302).
+__device__ __forceinline__ void neq_pi(const double n[9], int s, double &pxx,
+                                       double &pyy, double &pxy) {
+    switch (s) {
+    case UPPER:
+        pxx = n[1] + n[3] + 2. * (n[5] + n[6]);
+        pyy = 2. * (n[6] + n[2] + n[5]);
+        pxy = 2. * (n[5] - n[6]);
+        break;
+    case LOWER:
+        pxx = n[1] + n[3] + 2. * (n[7] + n[8]);
+        pyy = 2. * (n[4] + n[7] + n[8]);
+        pxy = 2. * (n[7] - n[8]);
+        break;
+    case RIGHT:
+        pxx = 2. * (n[1] + n[5] + n[8]);
+        pyy = n[2] + n[4] + 2. * (n[5] + n[8]);
+        pxy = 2. * (n[5] - n[8]);
+        break;
+    default:
+        pxx = 2. * (n[3] + n[6] + n[7]);
+        pyy = n[2] + n[4] + 2. * (n[6] + n[7]);
+        pxy = 2. * (n[7] - n[6]);
+        break;
+    }
+}
+
+// Regularized reconstruction (splitEqNeq + xNeqPi + regularizedF,
+// boundaries.c:239-262): f_i = feq_i + 9/2 w_i ((cx^2-1/3) Pxx +
+// (cy^2-1/3) Pyy + 2 cx cy Pxy), the bracket summed left to right.
+template <int I>
+__device__ __forceinline__ double reg_term(double pxx, double pyy, double pxy) {
+    constexpr double qx = (double)(cx(I) * cx(I)) - 1. / 3.;
+    constexpr double qy = (double)(cy(I) * cy(I)) - 1. / 3.;
+    constexpr double qxy = 2. * cx(I) * cy(I);
+    return 9. / 2. * wgt(I) * (qx * pxx + qy * pyy + qxy * pxy);
+}
+
+__device__ __forceinline__ void regularize(double f[9], int s, double rho, double ux,
+                                           double uy) {
+    double usqr = ux * ux + uy * uy;
+    double eq[9], neq[9];
+    for_pops([&]<int I>() {
+        eq[I] = feq<I>(rho, ux, uy, usqr);
+        neq[I] = f[I] - eq[I];
+    });
+    double pxx, pyy, pxy;
+    neq_pi(neq, s, pxx, pyy, pxy);
+    for_pops([&]<int I>() { f[I] = eq[I] + reg_term<I>(pxx, pyy, pxy); });
+}
+
+// Wall operator families and kinds of a channel side.
+enum BcFamily : int { BC_REGULARIZED = 0, BC_ZOUHE = 1 };
+enum BcKind : int { BC_VELOCITY = 0, BC_PRESSURE = 1 };
+
+// One straight-wall node: the operator of (side, kind, family) followed by
+// the bulk BGK, as every boundaries.c operator ends (e.g. boundaries.c:314).
+// p0/p1 are (ux, uy) for a velocity operator, (rho, uPar) for a pressure one.
+__device__ __forceinline__ void wall_node(double f[9], int s, int kind, int family,
+                                          double p0, double p1, double omega) {
+    bool normal_y = (s == UPPER || s == LOWER);
+    if (kind == BC_VELOCITY) {
+        double rho = wall_rho(f, s, normal_y ? p1 : p0);
+        if (family == BC_ZOUHE) zh_complete(f, s, p0, p1, rho);   // boundaries.c:171-201
+        else regularize(f, s, rho, p0, p1);                        // boundaries.c:304-358
+    } else {
+        double un = wall_un(f, s, p0);
+        double ux = normal_y ? p1 : un, uy = normal_y ? un : p1;
+        if (family == BC_ZOUHE) zh_complete(f, s, ux, uy, p0);    // boundaries.c:203-233
+        else regularize(f, s, p0, ux, uy);                         // boundaries.c:360-414
+    }
+    bgk(f, omega);
+}
+
+}  // namespace lbm

@@ -1,0 +1,295 @@
+// kernels_f64.hip -- FP64 collide/stream kernels for gfx950 (compiled with
+// -ffp-contract=off: the results are bit-identical to the reference C code).
+//
+// State conventions (both hold the reference's step semantics exactly):
+//  * streamed: buffer L holds the post-stream, pre-collision populations of
+//    step m on every interior node -- including, on perimeter nodes, the
+//    "unknown" populations whose source lies outside the domain; those hold
+//    whatever the reference's push left there: the node's own post-collision
+//    value of step m-2 (lb.c:258-271 never writes them).  Buffer T holds the
+//    post-collision populations of step m-1.  This is exactly the reference's
+//    (Simulation.lattice, Simulation.tmpLattice) pair after a step.
+//  * collided: buffer P holds the post-collision populations of step m-1 on
+//    interior nodes, and its ghost ring holds the unknown populations of the
+//    perimeter nodes for step m: ghost (x - c_i) slot i = post-collision of
+//    node x at step m-2.  A pull gather  f_i(x) = P(x - c_i)_i  then yields
+//    the reference's post-stream populations of step m on every node,
+//    perimeter included, with no branch.
+//
+// Kernels:
+//  k_collide<false>  streamed -> collided: reads L locally, collides, writes
+//                    the other buffer (one step's collision).
+//  k_collide<true>   collided -> collided: pull gather + collide (one fused
+//                    step), the combine/tight hot kernel.
+//  k_stream          collided -> streamed: pull gather only.
+//  k_init            equilibrium initial condition.
+//  k_macros / k_diag moments and reductions of the post-stream state.
+#include "launch.hpp"
+#include "d2q9_fp64.hpp"
+
+namespace lbm {
+
+__device__ __forceinline__ bool outside(const Layout &L, int x, int y) {
+    return x < 1 || x > L.lx || y < 1 || y > L.ly;
+}
+
+// Post-stream density of interior node (x, y): gathered from a collided
+// buffer, or read in place from a streamed one.
+template <bool PULL>
+__device__ __forceinline__ double density_at(const Layout &L, const double *__restrict__ s,
+                                             int x, int y) {
+    double g[9];
+    int64_t n = L.node(x, y);
+    for_pops([&]<int I>() {
+        g[I] = PULL ? s[I * L.plane + n - cx(I) * L.pitch - cy(I)] : s[I * L.plane + n];
+    });
+    return rho_of(g);
+}
+
+// Collision of one perimeter node, with the stale-slot bookkeeping and the
+// zero-gradient outlet density.  Kept out of line: perimeter nodes are a
+// thin minority and the bulk path should stay small.
+template <bool PULL>
+__device__ __noinline__ void perimeter_node(const Problem &p, int s, int x, int y,
+                                            double f[9], const double *__restrict__ src,
+                                            double *__restrict__ dst) {
+    const Layout &L = p.lay;
+    int64_t n = L.node(x, y);
+    // Populations entering this node from outside the domain must be present
+    // again at the next step (the reference reads them at the four corners).
+    // Collided state: keep them in dst's ghost ring = this node's value of
+    // the step before (src(x)_i).  Streamed state: that value sits in dst(x)
+    // itself and is read before dst(x) is overwritten below.
+    for_pops([&]<int I>() {
+        if constexpr (I != 0) {
+            int gx = x - cx(I), gy = y - cy(I);
+            if (outside(L, gx, gy)) {
+                double prev = PULL ? src[I * L.plane + n] : dst[I * L.plane + n];
+                dst[I * L.plane + L.node(gx, gy)] = prev;
+            }
+        }
+    });
+    const SideBC &b = p.side[s];
+    int pos = (s == UPPER || s == LOWER) ? x : y;
+    double p0 = b.p0[pos], p1 = b.p1[pos];
+    if (s == RIGHT && p.zgb) {
+        // updateZeroGradientBoundary (unsteady.c:205-214) on the post-stream
+        // state of this same step.
+        double rho1 = density_at<PULL>(L, src, L.lx - 1, y);
+        double rho2 = density_at<PULL>(L, src, L.lx - 2, y);
+        p0 = 4. / 3. * rho1 - 1. / 3. * rho2;
+    }
+    if (s == RIGHT) p.outlet_rho[y] = p0;
+    wall_node(f, s, b.kind, b.family, p0, p1, p.omega);
+}
+
+template <bool PULL>
+__global__ __launch_bounds__(256) void k_collide(Problem p, const double *__restrict__ src,
+                                                 double *__restrict__ dst) {
+    const Layout &L = p.lay;
+    int y = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int x = blockIdx.y + 1;
+    if (y > L.ly) return;
+    int64_t n = L.node(x, y);
+    double f[9];
+    for_pops([&]<int I>() {
+        f[I] = PULL ? src[I * L.plane + n - cx(I) * L.pitch - cy(I)] : src[I * L.plane + n];
+    });
+    int s = side_of(p, x, y);
+    if (s >= 0) perimeter_node<PULL>(p, s, x, y, f, src, dst);
+    else if (p.r2 >= 0 && in_disc(p, x, y)) bounce_back(f);
+    else bgk(f, p.omega);
+    for_pops([&]<int I>() { __builtin_nontemporal_store(f[I], &dst[I * L.plane + n]); });
+}
+
+__global__ __launch_bounds__(256) void k_stream(Layout L, const double *__restrict__ src,
+                                                double *__restrict__ dst) {
+    int y = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int x = blockIdx.y + 1;
+    if (y > L.ly) return;
+    int64_t n = L.node(x, y);
+    for_pops([&]<int I>() {
+        dst[I * L.plane + n] = src[I * L.plane + n - cx(I) * L.pitch - cy(I)];
+    });
+}
+
+// iniEquilibrium(node, 1, init_ux[y], 0) on every interior node
+// (unsteady.c:164-170, lb.c:71-78).
+__global__ void k_init(Layout L, const double *__restrict__ init_ux, double *__restrict__ dst) {
+    int y = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int x = blockIdx.y + 1;
+    if (y > L.ly) return;
+    double u = init_ux[y];
+    double usqr = u * u + 0. * 0.;
+    int64_t n = L.node(x, y);
+    for_pops([&]<int I>() { dst[I * L.plane + n] = feq<I>(1., u, 0., usqr); });
+}
+
+template <bool PULL>
+__global__ void k_macros(Layout L, const double *__restrict__ src, double *rho, double *ux,
+                         double *uy) {
+    int y = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int x = blockIdx.y + 1;
+    if (y > L.ly) return;
+    int64_t n = L.node(x, y);
+    double f[9];
+    for_pops([&]<int I>() {
+        f[I] = PULL ? src[I * L.plane + n - cx(I) * L.pitch - cy(I)] : src[I * L.plane + n];
+    });
+    double r, u, v;
+    macros(f, r, u, v);
+    int64_t o = (int64_t)(x - 1) * L.ly + (y - 1);
+    if (rho) rho[o] = r;
+    if (ux) ux[o] = u;
+    if (uy) uy[o] = v;
+}
+
+// ---- diagnostics: block partial sums, then one ordered final sum --------
+
+__device__ __forceinline__ bool solid_node(const Problem &p, int x, int y) {
+    return p.r2 >= 0 && side_of(p, x, y) < 0 && in_disc(p, x, y);
+}
+
+template <bool PULL>
+__global__ __launch_bounds__(256) void k_diag(Problem p, const double *__restrict__ src,
+                                              double *__restrict__ partial) {
+    const Layout &L = p.lay;
+    int y = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int x = blockIdx.y + 1;
+    double acc[4] = {0., 0., 0., 0.};
+    if (y <= L.ly) {
+        int64_t n = L.node(x, y);
+        double f[9];
+        for_pops([&]<int I>() {
+            f[I] = PULL ? src[I * L.plane + n - cx(I) * L.pitch - cy(I)] : src[I * L.plane + n];
+        });
+        bool solid = solid_node(p, x, y);
+        if (!solid) {
+            double r, u, v;
+            macros(f, r, u, v);
+            acc[2] = 0.5 * r * (u * u + v * v);
+            acc[3] = r;
+        }
+        // momentum exchange across solid / non-solid links
+        for_pops([&]<int I>() {
+            if constexpr (I != 0) {
+                int sx = x - cx(I), sy = y - cy(I);
+                if (!outside(L, sx, sy)) {
+                    bool from_solid = solid_node(p, sx, sy);
+                    if (solid && !from_solid) { acc[0] += cx(I) * f[I]; acc[1] += cy(I) * f[I]; }
+                    if (!solid && from_solid) { acc[0] -= cx(I) * f[I]; acc[1] -= cy(I) * f[I]; }
+                }
+            }
+        });
+    }
+    // wave64 tree reduction, then across the block's waves in LDS
+    __shared__ double red[4][4];
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int k = 0; k < 4; ++k) {
+        double v = acc[k];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (lane == 0) red[wave][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        int nw = (blockDim.x + 63) >> 6;
+        double v = 0.;
+        for (int w = 0; w < nw; ++w) v += red[w][threadIdx.x];
+        int64_t b = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+        partial[b * 4 + threadIdx.x] = v;
+    }
+}
+
+__global__ void k_diag_final(const double *__restrict__ partial, int64_t nblocks,
+                             double *__restrict__ out) {
+    // one wave per quantity; lane-strided partial sums then a tree
+    int k = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double v = 0.;
+    for (int64_t b = lane; b < nblocks; b += 64) v += partial[b * 4 + k];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (lane == 0) out[k] = v;
+}
+
+// ---- node-operator known answers -----------------------------------------
+
+__global__ void k_node_op(int which, int n, double *f, const double *p0, const double *p1,
+                          double omega) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    double g[9];
+    for (int i = 0; i < 9; ++i) g[i] = f[k * 9 + i];
+    if (which == 0) bounce_back(g);
+    else if (which == 17) bgk(g, omega);
+    else {
+        // 1..8 Zou/He, 9..16 regularized; within each: 4 velocity then 4
+        // pressure operators, sides in the order upper, lower, left, right.
+        int fam = which >= 9 ? BC_REGULARIZED : BC_ZOUHE;
+        int r = (which - 1) % 8;
+        int kind = r >= 4 ? BC_PRESSURE : BC_VELOCITY;
+        wall_node(g, r & 3, kind, fam, p0[k], p1[k], omega);
+    }
+    for (int i = 0; i < 9; ++i) f[k * 9 + i] = g[i];
+}
+
+// ---- launchers -----------------------------------------------------------
+
+static dim3 node_grid(const Layout &L, int &threads) {
+    threads = L.ly >= 256 ? 256 : ((L.ly + 63) / 64) * 64;
+    return dim3((L.ly + threads - 1) / threads, L.lx);
+}
+
+hipError_t launch_collide_f64(const Problem &p, bool pull, const double *src, double *dst,
+                              hipStream_t st) {
+    int t;
+    dim3 g = node_grid(p.lay, t);
+    if (pull) k_collide<true><<<g, t, 0, st>>>(p, src, dst);
+    else k_collide<false><<<g, t, 0, st>>>(p, src, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_stream_f64(const Layout &L, const double *src, double *dst, hipStream_t st) {
+    int t;
+    dim3 g = node_grid(L, t);
+    k_stream<<<g, t, 0, st>>>(L, src, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_f64(const Layout &L, const double *init_ux, double *dst, hipStream_t st) {
+    int t;
+    dim3 g = node_grid(L, t);
+    k_init<<<g, t, 0, st>>>(L, init_ux, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_macros_f64(const Layout &L, bool pull, const double *src, double *rho,
+                             double *ux, double *uy, hipStream_t st) {
+    int t;
+    dim3 g = node_grid(L, t);
+    if (pull) k_macros<true><<<g, t, 0, st>>>(L, src, rho, ux, uy);
+    else k_macros<false><<<g, t, 0, st>>>(L, src, rho, ux, uy);
+    return hipGetLastError();
+}
+
+int64_t diag_partials_needed(const Layout &L) {
+    int t;
+    dim3 g = node_grid(L, t);
+    return (int64_t)g.x * g.y;
+}
+
+hipError_t launch_diag_f64(const Problem &p, bool pull, const double *src, double *partial,
+                           double *out, hipStream_t st) {
+    int t;
+    dim3 g = node_grid(p.lay, t);
+    if (pull) k_diag<true><<<g, t, 0, st>>>(p, src, partial);
+    else k_diag<false><<<g, t, 0, st>>>(p, src, partial);
+    k_diag_final<<<1, 256, 0, st>>>(partial, (int64_t)g.x * g.y, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_node_op_f64(int which, int n, double *f, const double *p0, const double *p1,
+                              double omega, hipStream_t st) {
+    k_node_op<<<(n + 63) / 64, 64, 0, st>>>(which, n, f, p0, p1, omega);
+    return hipGetLastError();
+}
+
+}  // namespace lbm

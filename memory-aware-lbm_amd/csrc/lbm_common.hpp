@@ -1,0 +1,81 @@
+// lbm_common.hpp -- device data layout and node classification shared by
+// every kernel of the library.
+//
+// Layout in HBM (structure of arrays, one plane per population):
+//   element (i, x, y) at  i*plane + x*pitch + y + yoff
+// with x = 0..lx+1 and y = 0..ly+1 (a one-node ghost ring, as the
+// reference's (lx+2)*(ly+2) buffers, lb.c:89-100).  y is the contiguous
+// dimension, like the reference's lattice[iX][iY].  yoff = ALIGN-1 places
+// y = 1 on an ALIGN-element boundary, and the pitch is a multiple of ALIGN,
+// so every row's interior starts on a 512-byte (FP64) boundary.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lbm {
+
+constexpr int ALIGN = 64;  // elements
+
+struct Layout {
+    int lx, ly;
+    int pitch;        // elements per x-row
+    int yoff;         // offset of y = 0 inside a row
+    int64_t plane;    // elements per population plane
+
+    __host__ __device__ int64_t at(int i, int x, int y) const {
+        return (int64_t)i * plane + (int64_t)x * pitch + y + yoff;
+    }
+    __host__ __device__ int64_t node(int x, int y) const {
+        return (int64_t)x * pitch + y + yoff;
+    }
+};
+
+inline Layout make_layout(int lx, int ly) {
+    Layout L;
+    L.lx = lx;
+    L.ly = ly;
+    L.yoff = ALIGN - 1;
+    int need = L.yoff + ly + 2;
+    L.pitch = (need + ALIGN - 1) / ALIGN * ALIGN;
+    L.plane = (int64_t)(lx + 2) * L.pitch;
+    return L;
+}
+
+// Node classes of the cylinder channel, resolved analytically from (x, y) so
+// the hot loop reads no class map (the reference dispatches through a
+// per-node Dynamics pointer instead, lb.c:125-128; its assignment order is
+// unsteady.c:162-199: disc, then rows 1 and ly, then columns 1 and lx).
+enum NodeClass : int { NC_BULK = 0, NC_SOLID = 1, NC_WALL = 2 };
+
+struct SideBC {
+    int kind;            // BcKind
+    int family;          // BcFamily
+    const double *p0;    // device, indexed by x (upper/lower) or y (left/right)
+    const double *p1;
+};
+
+struct Problem {
+    Layout lay;
+    int ox, oy, r2;      // disc; r2 < 0 means no disc
+    double omega;
+    SideBC side[4];
+    int zgb;             // right side density by extrapolation
+    double *outlet_rho;  // device, ly+2: outlet density used by the last step
+};
+
+__host__ __device__ inline bool in_disc(const Problem &p, int x, int y) {
+    int dx = x - p.ox, dy = y - p.oy;
+    return dx * dx + dy * dy <= p.r2;
+}
+
+// side of a perimeter node, or -1 for interior nodes
+__host__ __device__ inline int side_of(const Problem &p, int x, int y) {
+    if (y == 1) return 1;              // LOWER
+    if (y == p.lay.ly) return 0;       // UPPER
+    if (x == 1) return 2;              // LEFT
+    if (x == p.lay.lx) return 3;       // RIGHT
+    return -1;
+}
+
+}  // namespace lbm

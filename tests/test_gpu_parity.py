@@ -1,0 +1,140 @@
+"""FP64 parity of the HIP path (through the C-ABI) with the oracle and with the
+golden vectors written by the real reference.  The bar is bit-equality; the
+north-star tolerance (max-abs <= 1e-12 on f_i) is asserted as well so a
+failure reports by how much."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import lbm_amd as L
+from oracle_lib import Channel as OChannel
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+TOL = 1e-12
+
+
+def dev_channel(c: OChannel) -> L.Channel:
+    return L.Channel(c.lx, c.ly, c.obst_x, c.obst_y, c.obst_r, c.u_max, c.reynolds,
+                     c.bc_family, c.zgb)
+
+
+def inner(a):
+    return a[1:-1, 1:-1, :]
+
+
+def assert_same(got, want, what):
+    d = float(np.abs(inner(got) - inner(want)).max())
+    assert d <= TOL, f"{what}: max-abs {d:g} > {TOL:g}"
+    assert np.array_equal(inner(got), inner(want)), f"{what}: not bit-identical (max-abs {d:g})"
+
+
+@pytest.mark.parametrize("which", range(18))
+def test_node_operator_kat(which):
+    kat = np.load(os.path.join(GOLD, "kat.npz"))
+    if which == 17:
+        f = kat["bgk_in"]
+        om = kat["bgk_omega"]
+        got = np.stack([L.apply_node_op(17, f[k:k + 1], np.zeros(1), np.zeros(1), om[k])[0]
+                        for k in range(len(f))])
+        assert np.array_equal(got, kat["bgk_out"])
+        return
+    f = kat["bc_in"][which]
+    p = kat["bc_p"][which]
+    om = kat["bc_omega"][which]
+    got = np.stack([L.apply_node_op(which, f[k:k + 1], p[k:k + 1, 0], p[k:k + 1, 1], om[k])[0]
+                    for k in range(len(f))])
+    want = kat["bc_out"][which]
+    assert np.abs(got - want).max() <= TOL
+    assert np.array_equal(got, want), str(kat["bc_names"][which])
+
+
+@pytest.mark.parametrize("name", ["traj_32x16_reg.npz", "traj_32x16_zh.npz"])
+@pytest.mark.parametrize("variant", ["origin", "combine", "tight"])
+def test_trajectory_golden(name, variant):
+    g = np.load(os.path.join(GOLD, name))
+    ch = OChannel(**json.loads(str(g["channel"])))
+    for s in (1, 2, 3, 10, 200):
+        with L.Lattice(dev_channel(ch)) as lat:
+            lat.init_equilibrium()
+            lat.step(s, variant)
+            lo, to, po = lat.download()
+        assert_same(lo, g[f"lat_{s}"], f"{name} {variant} lattice after {s}")
+        assert_same(to, g[f"tmp_{s}"], f"{name} {variant} tmpLattice after {s}")
+        assert np.array_equal(po[2:ch.ly], g[f"prho_{s}"][2:ch.ly])
+
+
+@pytest.mark.parametrize("geom", ["A", "B"])
+@pytest.mark.parametrize("variant", ["origin", "combine"])
+def test_c1_parity_2000_steps(orc, geom, variant):
+    """256x128, Re 100, 2000 steps: the parity configuration."""
+    ch = OChannel.geometry(256, 128, geom)
+    ran, lat_ref, tmp_ref, prho_ref = orc.run(ch, 2000, "origin")
+    with L.Lattice(dev_channel(ch)) as lat:
+        lat.init_equilibrium()
+        lat.step(2000, variant)
+        lo, to, po = lat.download()
+        rho, ux, uy = lat.macros()
+    assert_same(lo, lat_ref, f"C1-{geom} {variant} lattice")
+    assert_same(to, tmp_ref, f"C1-{geom} {variant} tmpLattice")
+    assert np.array_equal(po[2:128], prho_ref[2:128])
+    if geom == "B":
+        fl = np.load(os.path.join(GOLD, "c1_B_fields.npz"))
+        assert np.array_equal(rho, fl["rho"])
+        assert np.array_equal(ux, fl["ux"])
+        assert np.array_equal(uy, fl["uy"])
+
+
+def test_resume_from_uploaded_state(orc):
+    ch = OChannel.geometry(64, 32, "B")
+    _, l7, t7, p7 = orc.run(ch, 7)
+    _, l20, t20, p20 = orc.run(ch, 20)
+    with L.Lattice(dev_channel(ch)) as lat:
+        lat.upload(l7, t7, p7)
+        lat.step(13, "combine")
+        lo, to, po = lat.download()
+    assert_same(lo, l20, "resumed lattice")
+    assert_same(to, t20, "resumed tmpLattice")
+
+
+def test_mixed_variants_match(orc):
+    """Variants can be switched between calls without changing the result."""
+    ch = OChannel.geometry(48, 24, "A")
+    _, lref, tref, _ = orc.run(ch, 15)
+    with L.Lattice(dev_channel(ch)) as lat:
+        lat.init_equilibrium()
+        for n, v in ((3, "origin"), (4, "combine"), (2, "origin"), (1, "tight"), (5, "origin")):
+            lat.step(n, v)
+        assert lat.steps_done() == 15
+        lo, to, _ = lat.download()
+    assert_same(lo, lref, "mixed lattice")
+    assert_same(to, tref, "mixed tmpLattice")
+
+
+def test_download_is_repeatable():
+    ch = OChannel.geometry(40, 20, "B")
+    with L.Lattice(dev_channel(ch)) as lat:
+        lat.init_equilibrium()
+        lat.step(9, "combine")
+        a = lat.download()
+        b = lat.download()
+        lat.step(1, "combine")
+        c = lat.download()
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert not np.array_equal(a[0], c[0])
+
+
+def test_diagnostics_match_oracle(orc):
+    ch = OChannel.geometry(128, 64, "B")
+    _, lref, _, _ = orc.run(ch, 300)
+    want = orc.diag(ch, lref)
+    with L.Lattice(dev_channel(ch)) as lat:
+        lat.init_equilibrium()
+        lat.step(300, "combine")
+        got = lat.diagnostics()
+    for k in range(4):
+        scale = max(abs(want[k]), 1e-300)
+        assert abs(got[k] - want[k]) <= 1e-12 * max(scale, 1e-3), (k, got[k], want[k])
