@@ -70,6 +70,7 @@ def main():
         except lbm.LbmError as e:
             others[v] = {"error": str(e)}
     achieved = head["gbs"]
+    copy_gbs = lbm.copy_bandwidth(2 << 30, 20)
     out = {
         "metric": METRIC,
         "value": round(head["glups"], 4),
@@ -90,6 +91,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None},
         "variants": others,
+        "copy_kernel_gbs": round(copy_gbs, 1),
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)

@@ -175,6 +175,11 @@ size_t lbm_hip_footprint_of(const lbm_problem *prob);
 int lbm_hip_apply_node_op(int which, int n, double *f, const double *p0,
                           const double *p1, double omega);
 
+/* Measurement support (no counterpart in the reference): time a plain
+ * device-to-device copy kernel of `bytes` (read once, written once) over
+ * iters launches; the HBM yardstick the roofline figures are read against. */
+int lbm_hip_copy_bandwidth(size_t bytes, int iters, double *ms_per_copy);
+
 /* Description of the last error on this thread. */
 const char *lbm_hip_last_error(void);
 

@@ -66,8 +66,8 @@ int validate(const lbm_problem *p) {
     if (p->lx < 3 || p->ly < 3) return fail(LBM_ERR_ARG, "lx and ly must be >= 3");
     if ((int64_t)(p->lx + 2) * (p->ly + 2) > (int64_t)1 << 31)
         return fail(LBM_ERR_ARG, "lattice too large for 32-bit row indexing");
-    if (!(p->omega > 0.0 && p->omega < 2.0))
-        return fail(LBM_ERR_ARG, "omega must lie in (0, 2)");
+    if (!(p->omega > 0.0 && p->omega <= 2.0))
+        return fail(LBM_ERR_ARG, "omega must lie in (0, 2]");
     for (int s = 0; s < 4; ++s) {
         const lbm_side_bc &b = p->side[s];
         if (b.kind != LBM_BC_VELOCITY && b.kind != LBM_BC_PRESSURE)
@@ -151,8 +151,39 @@ int step_impl(lbm_ctx *c, int variant, int nsteps) {
         return LBM_OK;
     }
     case LBM_TIGHT_BLOCK:
-    case LBM_PANEL:
-        return fail(LBM_ERR_UNSUPPORTED, "blocked two-step variants not built yet");
+    case LBM_PANEL: {
+        // two steps per launch (lb.c:936-1069, 1402-1538); an odd count ends
+        // with one fused single step
+        if (c->prob.lay.lx < 4)
+            return fail(LBM_ERR_UNSUPPORTED, "two-step variants need lx >= 4");
+        int k = 0;
+        if (!c->collided) {
+            int rc = to_collided(c);
+            if (rc) return rc;
+            k = 1;
+        }
+        for (; k + 2 <= nsteps; k += 2) {
+            if (variant == LBM_PANEL) {
+                PanelGrid g = panel_grid(c->prob.lay, c->tile_y, c->tile_x);
+                HIPCHK(launch_panel_f64(c->prob, g, dbuf(c, c->cur), dbuf(c, 1 - c->cur),
+                                        c->stream));
+            } else {
+                BlockGrid g = block_grid(c->prob.lay, c->tile_x,
+                                         c->tile_y > 0 ? c->tile_y + 2 : 0);
+                HIPCHK(launch_block_f64(c->prob, g, dbuf(c, c->cur), dbuf(c, 1 - c->cur),
+                                        c->stream));
+            }
+            c->cur = 1 - c->cur;
+            c->steps += 2;
+        }
+        if (k < nsteps) {
+            HIPCHK(launch_collide_f64(c->prob, true, dbuf(c, c->cur), dbuf(c, 1 - c->cur),
+                                      c->stream));
+            c->cur = 1 - c->cur;
+            c->steps += 1;
+        }
+        return LBM_OK;
+    }
     default:
         return fail(LBM_ERR_ARG, "unknown variant");
     }
@@ -219,6 +250,12 @@ int lbm_hip_create(const lbm_problem *p, lbm_ctx **out) {
     c->prob.omega = p->omega;
     c->prob.zgb = p->zgb;
     c->npartial = diag_partials_needed(c->prob.lay);
+    static bool lds_attr_set = false;
+    if (!lds_attr_set) {
+        hipError_t ea = block_kernel_setup();
+        if (ea != hipSuccess) { delete c; return hip_fail(ea, "hipFuncSetAttribute"); }
+        lds_attr_set = true;
+    }
     size_t bytes = 9 * (size_t)c->prob.lay.plane * elem_size(p->storage);
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete c; return hip_fail(e, "hipStreamCreate"); }
@@ -446,6 +483,32 @@ int lbm_hip_apply_node_op(int which, int n, double *f, const double *p0, const d
     if (e == hipSuccess) e = hipMemcpy(f, d, (size_t)n * 9 * sizeof(double), hipMemcpyDeviceToHost);
     (void)hipFree(d);
     if (e != hipSuccess) return hip_fail(e, "node operator");
+    return LBM_OK;
+}
+
+int lbm_hip_copy_bandwidth(size_t bytes, int iters, double *ms_per_copy) {
+    if (bytes < 16 || iters < 1 || !ms_per_copy) return fail(LBM_ERR_ARG, "bad copy request");
+    bytes &= ~(size_t)15;
+    void *a = nullptr, *b = nullptr;
+    HIPCHK(hipMalloc(&a, bytes));
+    hipError_t e = hipMalloc(&b, bytes);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (e == hipSuccess) e = hipMemset(a, 0, bytes);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = launch_copy(a, b, (int64_t)bytes, 0);  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(e0, 0);
+    for (int k = 0; k < iters && e == hipSuccess; ++k) e = launch_copy(a, b, (int64_t)bytes, 0);
+    if (e == hipSuccess) e = hipEventRecord(e1, 0);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (e != hipSuccess) return hip_fail(e, "copy bandwidth");
+    *ms_per_copy = ms / iters;
     return LBM_OK;
 }
 

@@ -100,83 +100,76 @@ __device__ __forceinline__ void bounce_back(double f[9]) {
     t = f[6]; f[6] = f[8]; f[8] = t;
 }
 
+// The wall operators below take the side as a template parameter: with a
+// run-time side the compiler turns the per-side population indices into a
+// lookup table and the population array into scratch memory.
+
 // Known-population sum at a wall node (boundaries.c:66-126 bracket it as
 // tangential triple plus twice the outgoing triple).
-__device__ __forceinline__ double wall_sum(const double f[9], int s) {
-    switch (s) {
-    case UPPER: return f[0] + f[3] + f[1] + 2 * (f[6] + f[2] + f[5]);
-    case LOWER: return f[0] + f[3] + f[1] + 2 * (f[8] + f[4] + f[7]);
-    case RIGHT: return f[0] + f[2] + f[4] + 2 * (f[1] + f[5] + f[8]);
-    default:    return f[0] + f[2] + f[4] + 2 * (f[3] + f[6] + f[7]);
-    }
+template <int S> __device__ __forceinline__ double wall_sum(const double f[9]) {
+    if constexpr (S == UPPER) return f[0] + f[3] + f[1] + 2 * (f[6] + f[2] + f[5]);
+    else if constexpr (S == LOWER) return f[0] + f[3] + f[1] + 2 * (f[8] + f[4] + f[7]);
+    else if constexpr (S == RIGHT) return f[0] + f[2] + f[4] + 2 * (f[1] + f[5] + f[8]);
+    else return f[0] + f[2] + f[4] + 2 * (f[3] + f[6] + f[7]);
 }
 
 // Density from a prescribed normal velocity (upperRho, lowerRho, rightRho,
 // leftRho: boundaries.c:66-118): reciprocal first, then the product.
-__device__ __forceinline__ double wall_rho(const double f[9], int s, double un) {
-    double sgn_den = (s == UPPER || s == RIGHT) ? 1. + un : 1. - un;
-    return 1. / sgn_den * wall_sum(f, s);
+template <int S> __device__ __forceinline__ double wall_rho(const double f[9], double un) {
+    double den = (S == UPPER || S == RIGHT) ? 1. + un : 1. - un;
+    return 1. / den * wall_sum<S>(f);
 }
 
 // Normal velocity from a prescribed density (upperU, lowerU, rightU, leftU:
 // boundaries.c:74-126).
-__device__ __forceinline__ double wall_un(const double f[9], int s, double rho) {
-    double q = 1. / rho * wall_sum(f, s);
-    return (s == UPPER || s == RIGHT) ? -1. + q : 1. - q;
+template <int S> __device__ __forceinline__ double wall_un(const double f[9], double rho) {
+    double q = 1. / rho * wall_sum<S>(f);
+    return (S == UPPER || S == RIGHT) ? -1. + q : 1. - q;
 }
 
 // Zou/He completion of the three unknown populations (boundaries.c:131-169).
-__device__ __forceinline__ void zh_complete(double f[9], int s, double ux, double uy,
-                                            double rho) {
-    switch (s) {
-    case UPPER:
+template <int S>
+__device__ __forceinline__ void zh_complete(double f[9], double ux, double uy, double rho) {
+    if constexpr (S == UPPER) {
         f[7] = f[5] + 0.5 * (f[1] - f[3]) - rho * uy / 6 - rho * ux / 2.;
         f[8] = f[6] + 0.5 * (f[3] - f[1]) - rho * uy / 6 + rho * ux / 2.;
         f[4] = f[2] - 2. / 3. * rho * uy;
-        break;
-    case LOWER:
+    } else if constexpr (S == LOWER) {
         f[6] = f[8] + 0.5 * (f[1] - f[3]) + rho * uy / 6 - rho * ux / 2.;
         f[5] = f[7] + 0.5 * (f[3] - f[1]) + rho * uy / 6 + rho * ux / 2.;
         f[2] = f[4] + 2. / 3. * rho * uy;
-        break;
-    case RIGHT:
+    } else if constexpr (S == RIGHT) {
         f[6] = f[8] + 0.5 * (f[4] - f[2]) - rho * ux / 6 + rho * uy / 2.;
         f[7] = f[5] + 0.5 * (f[2] - f[4]) - rho * ux / 6 - rho * uy / 2.;
         f[3] = f[1] - 2. / 3. * rho * ux;
-        break;
-    default:
+    } else {
         f[5] = f[7] + 0.5 * (f[4] - f[2]) + rho * ux / 6. + rho * uy / 2.;
         f[8] = f[6] + 0.5 * (f[2] - f[4]) + rho * ux / 6. - rho * uy / 2.;
         f[1] = f[3] + 2. / 3. * rho * ux;
-        break;
     }
 }
 
 // Non-equilibrium momentum flux from the known populations (xNeqPi,
 // boundaries.c:266-302).
-__device__ __forceinline__ void neq_pi(const double n[9], int s, double &pxx,
-                                       double &pyy, double &pxy) {
-    switch (s) {
-    case UPPER:
+template <int S>
+__device__ __forceinline__ void neq_pi(const double n[9], double &pxx, double &pyy,
+                                       double &pxy) {
+    if constexpr (S == UPPER) {
         pxx = n[1] + n[3] + 2. * (n[5] + n[6]);
         pyy = 2. * (n[6] + n[2] + n[5]);
         pxy = 2. * (n[5] - n[6]);
-        break;
-    case LOWER:
+    } else if constexpr (S == LOWER) {
         pxx = n[1] + n[3] + 2. * (n[7] + n[8]);
         pyy = 2. * (n[4] + n[7] + n[8]);
         pxy = 2. * (n[7] - n[8]);
-        break;
-    case RIGHT:
+    } else if constexpr (S == RIGHT) {
         pxx = 2. * (n[1] + n[5] + n[8]);
         pyy = n[2] + n[4] + 2. * (n[5] + n[8]);
         pxy = 2. * (n[5] - n[8]);
-        break;
-    default:
+    } else {
         pxx = 2. * (n[3] + n[6] + n[7]);
         pyy = n[2] + n[4] + 2. * (n[6] + n[7]);
         pxy = 2. * (n[7] - n[6]);
-        break;
     }
 }
 
@@ -191,8 +184,8 @@ __device__ __forceinline__ double reg_term(double pxx, double pyy, double pxy) {
     return 9. / 2. * wgt(I) * (qx * pxx + qy * pyy + qxy * pxy);
 }
 
-__device__ __forceinline__ void regularize(double f[9], int s, double rho, double ux,
-                                           double uy) {
+template <int S>
+__device__ __forceinline__ void regularize(double f[9], double rho, double ux, double uy) {
     double usqr = ux * ux + uy * uy;
     double eq[9], neq[9];
     for_pops([&]<int I>() {
@@ -200,7 +193,7 @@ __device__ __forceinline__ void regularize(double f[9], int s, double rho, doubl
         neq[I] = f[I] - eq[I];
     });
     double pxx, pyy, pxy;
-    neq_pi(neq, s, pxx, pyy, pxy);
+    neq_pi<S>(neq, pxx, pyy, pxy);
     for_pops([&]<int I>() { f[I] = eq[I] + reg_term<I>(pxx, pyy, pxy); });
 }
 
@@ -211,18 +204,19 @@ enum BcKind : int { BC_VELOCITY = 0, BC_PRESSURE = 1 };
 // One straight-wall node: the operator of (side, kind, family) followed by
 // the bulk BGK, as every boundaries.c operator ends (e.g. boundaries.c:314).
 // p0/p1 are (ux, uy) for a velocity operator, (rho, uPar) for a pressure one.
-__device__ __forceinline__ void wall_node(double f[9], int s, int kind, int family,
-                                          double p0, double p1, double omega) {
-    bool normal_y = (s == UPPER || s == LOWER);
+template <int S>
+__device__ __forceinline__ void wall_node(double f[9], int kind, int family, double p0,
+                                          double p1, double omega) {
+    constexpr bool normal_y = (S == UPPER || S == LOWER);
     if (kind == BC_VELOCITY) {
-        double rho = wall_rho(f, s, normal_y ? p1 : p0);
-        if (family == BC_ZOUHE) zh_complete(f, s, p0, p1, rho);   // boundaries.c:171-201
-        else regularize(f, s, rho, p0, p1);                        // boundaries.c:304-358
+        double rho = wall_rho<S>(f, normal_y ? p1 : p0);
+        if (family == BC_ZOUHE) zh_complete<S>(f, p0, p1, rho);   // boundaries.c:171-201
+        else regularize<S>(f, rho, p0, p1);                        // boundaries.c:304-358
     } else {
-        double un = wall_un(f, s, p0);
+        double un = wall_un<S>(f, p0);
         double ux = normal_y ? p1 : un, uy = normal_y ? un : p1;
-        if (family == BC_ZOUHE) zh_complete(f, s, ux, uy, p0);    // boundaries.c:203-233
-        else regularize(f, s, p0, ux, uy);                         // boundaries.c:360-414
+        if (family == BC_ZOUHE) zh_complete<S>(f, ux, uy, p0);    // boundaries.c:203-233
+        else regularize<S>(f, p0, ux, uy);                         // boundaries.c:360-414
     }
     bgk(f, omega);
 }

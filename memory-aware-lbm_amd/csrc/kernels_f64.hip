@@ -25,13 +25,9 @@
 //  k_init            equilibrium initial condition.
 //  k_macros / k_diag moments and reductions of the post-stream state.
 #include "launch.hpp"
-#include "d2q9_fp64.hpp"
+#include "collide_f64.hpp"
 
 namespace lbm {
-
-__device__ __forceinline__ bool outside(const Layout &L, int x, int y) {
-    return x < 1 || x > L.lx || y < 1 || y > L.ly;
-}
 
 // Post-stream density of interior node (x, y): gathered from a collided
 // buffer, or read in place from a streamed one.
@@ -50,7 +46,7 @@ __device__ __forceinline__ double density_at(const Layout &L, const double *__re
 // zero-gradient outlet density.  Kept out of line: perimeter nodes are a
 // thin minority and the bulk path should stay small.
 template <bool PULL>
-__device__ __noinline__ void perimeter_node(const Problem &p, int s, int x, int y,
+__device__ __forceinline__ void perimeter_node(const Problem &p, int s, int x, int y,
                                             double f[9], const double *__restrict__ src,
                                             double *__restrict__ dst) {
     const Layout &L = p.lay;
@@ -69,18 +65,15 @@ __device__ __noinline__ void perimeter_node(const Problem &p, int s, int x, int 
             }
         }
     });
-    const SideBC &b = p.side[s];
-    int pos = (s == UPPER || s == LOWER) ? x : y;
-    double p0 = b.p0[pos], p1 = b.p1[pos];
+    double right_p0 = 0.;
     if (s == RIGHT && p.zgb) {
         // updateZeroGradientBoundary (unsteady.c:205-214) on the post-stream
         // state of this same step.
-        double rho1 = density_at<PULL>(L, src, L.lx - 1, y);
-        double rho2 = density_at<PULL>(L, src, L.lx - 2, y);
-        p0 = 4. / 3. * rho1 - 1. / 3. * rho2;
+        right_p0 = zgb_rho(density_at<PULL>(L, src, L.lx - 1, y),
+                           density_at<PULL>(L, src, L.lx - 2, y));
     }
-    if (s == RIGHT) p.outlet_rho[y] = p0;
-    wall_node(f, s, b.kind, b.family, p0, p1, p.omega);
+    if (s == RIGHT) p.outlet_rho[y] = p.zgb ? right_p0 : p.side[RIGHT].p0[y];
+    collide_class(p, s, x, y, f, right_p0);
 }
 
 template <bool PULL>
@@ -226,9 +219,25 @@ __global__ void k_node_op(int which, int n, double *f, const double *p0, const d
         int fam = which >= 9 ? BC_REGULARIZED : BC_ZOUHE;
         int r = (which - 1) % 8;
         int kind = r >= 4 ? BC_PRESSURE : BC_VELOCITY;
-        wall_node(g, r & 3, kind, fam, p0[k], p1[k], omega);
+        switch (r & 3) {
+        case UPPER: wall_node<UPPER>(g, kind, fam, p0[k], p1[k], omega); break;
+        case LOWER: wall_node<LOWER>(g, kind, fam, p0[k], p1[k], omega); break;
+        case LEFT: wall_node<LEFT>(g, kind, fam, p0[k], p1[k], omega); break;
+        default: wall_node<RIGHT>(g, kind, fam, p0[k], p1[k], omega); break;
+        }
     }
     for (int i = 0; i < 9; ++i) f[k * 9 + i] = g[i];
+}
+
+// ---- plain copy: the bandwidth yardstick for the roofline ------------------
+
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+
+__global__ __launch_bounds__(256) void k_copy(const dbl2 *__restrict__ src,
+                                              dbl2 *__restrict__ dst, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) __builtin_nontemporal_store(src[i], &dst[i]);
 }
 
 // ---- launchers -----------------------------------------------------------
@@ -283,6 +292,14 @@ hipError_t launch_diag_f64(const Problem &p, bool pull, const double *src, doubl
     if (pull) k_diag<true><<<g, t, 0, st>>>(p, src, partial);
     else k_diag<false><<<g, t, 0, st>>>(p, src, partial);
     k_diag_final<<<1, 256, 0, st>>>(partial, (int64_t)g.x * g.y, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t st) {
+    int64_t n = bytes / 16;
+    int blocks = (int)((n + 255) / 256 < 256 * 1024 ? (n + 255) / 256 : 256 * 1024);
+    if (blocks < 1) blocks = 1;
+    k_copy<<<blocks, 256, 0, st>>>((const dbl2 *)src, (dbl2 *)dst, n);
     return hipGetLastError();
 }
 

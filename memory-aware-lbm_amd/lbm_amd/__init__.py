@@ -79,6 +79,7 @@ def library():
         "lbm_hip_footprint": (C.c_size_t, [vp]),
         "lbm_hip_footprint_of": (C.c_size_t, [C.POINTER(_Problem)]),
         "lbm_hip_apply_node_op": (C.c_int, [C.c_int, C.c_int, _P, _P, _P, C.c_double]),
+        "lbm_hip_copy_bandwidth": (C.c_int, [C.c_size_t, C.c_int, _P]),
         "lbm_hip_last_error": (C.c_char_p, []),
     }
     for name, (res, args) in sig.items():
@@ -244,6 +245,13 @@ class Lattice:
 
     def footprint(self) -> int:
         return int(library().lbm_hip_footprint(self._h))
+
+
+def copy_bandwidth(nbytes: int, iters: int = 20) -> float:
+    """GB/s of a plain device copy kernel moving nbytes in and nbytes out."""
+    ms = C.c_double()
+    _check(library().lbm_hip_copy_bandwidth(int(nbytes), int(iters), C.byref(ms)), "copy")
+    return 2.0 * nbytes / (ms.value * 1e-3) / 1e9
 
 
 def apply_node_op(which: int, f: np.ndarray, p0: np.ndarray, p1: np.ndarray,

@@ -53,7 +53,7 @@ def test_node_operator_kat(which):
 
 
 @pytest.mark.parametrize("name", ["traj_32x16_reg.npz", "traj_32x16_zh.npz"])
-@pytest.mark.parametrize("variant", ["origin", "combine", "tight"])
+@pytest.mark.parametrize("variant", ["origin", "combine", "tight", "tight_block", "panel"])
 def test_trajectory_golden(name, variant):
     g = np.load(os.path.join(GOLD, name))
     ch = OChannel(**json.loads(str(g["channel"])))
@@ -68,7 +68,7 @@ def test_trajectory_golden(name, variant):
 
 
 @pytest.mark.parametrize("geom", ["A", "B"])
-@pytest.mark.parametrize("variant", ["origin", "combine"])
+@pytest.mark.parametrize("variant", ["origin", "combine", "tight_block", "panel"])
 def test_c1_parity_2000_steps(orc, geom, variant):
     """256x128, Re 100, 2000 steps: the parity configuration."""
     ch = OChannel.geometry(256, 128, geom)
@@ -138,3 +138,24 @@ def test_diagnostics_match_oracle(orc):
     for k in range(4):
         scale = max(abs(want[k]), 1e-300)
         assert abs(got[k] - want[k]) <= 1e-12 * max(scale, 1e-3), (k, got[k], want[k])
+
+
+@pytest.mark.parametrize("variant,tiles", [
+    ("panel", [(0, 0), (3, 5), (7, 254), (130, 17), (4, 1)]),
+    ("tight_block", [(0, 0), (3, 30), (5, 62), (20, 126), (50, 62)]),
+])
+def test_two_step_tile_shapes(orc, variant, tiles):
+    """Bit-identical for every tile / panel shape, including tiles that do not
+    divide the lattice and lattices smaller than one tile."""
+    for lx, ly, geom in ((61, 37, "B"), (23, 100, "A"), (4, 9, "A")):
+        ch = OChannel.geometry(lx, ly, geom)
+        _, lref, tref, pref = orc.run(ch, 13)
+        for tx, ty in tiles:
+            with L.Lattice(dev_channel(ch)) as lat:
+                lat.set_tile(tx, ty)
+                lat.init_equilibrium()
+                lat.step(13, variant)
+                lo, to, po = lat.download()
+            assert_same(lo, lref, f"{variant} {lx}x{ly} tile {tx}x{ty}")
+            assert_same(to, tref, f"{variant} {lx}x{ly} tile {tx}x{ty} tmp")
+            assert np.array_equal(po[2:ly], pref[2:ly])
