@@ -126,6 +126,15 @@ int lbm_hip_download(lbm_ctx *ctx, double *lattice, double *tmp, double *outlet_
  * the context's stream; returns without waiting. */
 int lbm_hip_step(lbm_ctx *ctx, int variant, int nsteps);
 
+/* The origin sweep split in its two halves, for callers that drive them
+ * separately as the reference driver does.  lbm_hip_collide performs one
+ * collision (collide / collide_openmp, lb.c:172-243) and needs the streamed
+ * state; lbm_hip_propagate performs the streaming (propagate /
+ * propagate_openmp, lb.c:246-341) and needs the collided state.  A call in the
+ * wrong state returns LBM_ERR_STATE. */
+int lbm_hip_collide(lbm_ctx *ctx);
+int lbm_hip_propagate(lbm_ctx *ctx);
+
 /* Block-size knobs of the blocked variants (the reference's blk_size /
  * thread_block, unsteady.c:93,258): tile extent in x and y.  0 = default. */
 int lbm_hip_set_tile(lbm_ctx *ctx, int tile_x, int tile_y);

@@ -379,6 +379,18 @@ int lbm_hip_step(lbm_ctx *c, int variant, int nsteps) {
     return step_impl(c, variant, nsteps);
 }
 
+int lbm_hip_collide(lbm_ctx *c) {
+    if (!c) return fail(LBM_ERR_ARG, "null context");
+    if (c->collided) return fail(LBM_ERR_STATE, "collide needs the streamed state (propagate first)");
+    return to_collided(c);
+}
+
+int lbm_hip_propagate(lbm_ctx *c) {
+    if (!c) return fail(LBM_ERR_ARG, "null context");
+    if (!c->collided) return fail(LBM_ERR_STATE, "propagate needs a collided state (collide first)");
+    return to_streamed(c);
+}
+
 int lbm_hip_set_tile(lbm_ctx *c, int tile_x, int tile_y) {
     if (!c || tile_x < 0 || tile_y < 0) return fail(LBM_ERR_ARG, "bad tile");
     c->tile_x = tile_x;

@@ -1,0 +1,55 @@
+"""The reference command line (unsteady_<variant> lx ly blk_size thread_block
+with STOP) driving the GPU through the host lb.h mirror: the Dynamics table
+is translated by function identity, the sweeps run on the device, and the
+final (lattice, tmpLattice) equals the oracle's bit for bit."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle_lib import Channel
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "memory-aware-lbm_amd", "unsteady_gpu")
+
+
+def run_driver(lx, ly, steps, variant, tmp_path, disc="A", family="regularized"):
+    dump = str(tmp_path / f"dump_{variant}.bin")
+    env = dict(os.environ, STOP=str(steps), LBM_DUMP=dump, LBM_DISC=disc, LBM_FAMILY=family)
+    r = subprocess.run([EXE, str(lx), str(ly), "8", "16", variant], env=env,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "time eclapsed in" in r.stdout
+    raw = np.fromfile(dump, dtype=np.float64).reshape(2, lx + 2, ly + 2, 9)
+    return raw[0], raw[1], r.stdout
+
+
+@pytest.mark.parametrize("variant", ["origin", "combine", "tight", "tight_block", "panel"])
+@pytest.mark.parametrize("family", ["regularized", "zouhe"])
+def test_driver_matches_oracle(orc, tmp_path, variant, family):
+    lx, ly, steps = 96, 40, 30
+    lat, tmp, out = run_driver(lx, ly, steps, variant, tmp_path, family=family)
+    ch = Channel.geometry(lx, ly, "A", bc_family=1 if family == "zouhe" else 0)
+    _, lref, tref, _ = orc.run(ch, steps)
+    assert np.array_equal(lat[1:-1, 1:-1], lref[1:-1, 1:-1]), out
+    assert np.array_equal(tmp[1:-1, 1:-1], tref[1:-1, 1:-1]), out
+
+
+def test_driver_savevel_format(orc, tmp_path):
+    lx, ly = 40, 20
+    path = tmp_path / "vel.dat"
+    env = dict(os.environ, STOP="10", LBM_SAVEVEL=str(path))
+    r = subprocess.run([EXE, str(lx), str(ly), "8"], env=env, capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 0, r.stderr
+    rows = path.read_text().splitlines()
+    assert len(rows) == lx * ly
+    _, lref, _, _ = orc.run(Channel.geometry(lx, ly, "A"), 10)
+    # saveVel order: y outer, x inner; "%f,%f,%f" of (ux, 1.0, uy) (lb.c:1841-1846)
+    for k in (0, 7, lx * ly // 2, lx * ly - 1):
+        y, x = divmod(k, lx)
+        rho, ux, uy = orc.macros(lref[x + 1, y + 1])
+        assert rows[k] == f"{ux:f},{1.0:f},{uy:f}"
