@@ -55,14 +55,16 @@ def parse():
     ap.add_argument("--variants", default="origin,combine,panel",
                     help="variants timed besides the headline one (comma list, or '')")
     ap.add_argument("--other-steps", type=int, default=200)
+    ap.add_argument("--storages", default="f32,f16",
+                    help="reduced-precision storage forms timed with the fused kernel")
     ap.add_argument("--cpu-baseline", default="auto", choices=["auto", "off"])
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--cpu-threads", type=int, default=16)
     return ap.parse_args()
 
 
-def time_variant(lbm, ch, variant, steps, warmup):
-    with lbm.Lattice(ch) as lat:
+def time_variant(lbm, ch, variant, steps, warmup, storage="f64"):
+    with lbm.Lattice(ch, storage=storage) as lat:
         lat.init_equilibrium()
         lat.step(max(warmup, 1), variant)   # leaves the state in the variant's convention
         lat.sync()
@@ -144,6 +146,14 @@ def main():
         others[v] = {"glups": round(r["glups"], 4), "ms_per_step": round(r["ms_per_step"], 5),
                      "kernel": KERNEL_OF[v], "launch_ms": round(r["launch_ms"], 5),
                      "achieved_gbs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4)}
+    storage = {}
+    for sname in [v for v in args.storages.split(",") if v]:
+        r = time_variant(lbm, ch, "combine", args.other_steps, min(args.warmup, 20), sname)
+        bpu = {"f32": 72.0, "f16": 36.0}[sname]
+        ach = bpu * cells / (r["launch_ms"] * 1e-3) / 1e9
+        storage[sname] = {"glups": round(r["glups"], 4), "ms_per_step": round(r["ms_per_step"], 5),
+                          "kernel": f"k_collide_lp<{sname}>", "bytes_per_update": bpu,
+                          "achieved_gbs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4)}
     copy_gbs = lbm.copy_bandwidth(2 << 30, 20)
     kernel = KERNEL_OF[args.variant]
     upl = UPDATES_PER_LAUNCH[args.variant]
@@ -174,6 +184,7 @@ def main():
                      "hbm_frac_measured": (round(traffic / (head["launch_ms"] * 1e-3) / 1e9
                                                  / HBM_PEAK_GBS, 4) if traffic else None)},
         "variants": others,
+        "storage_forms": storage,
         "copy_kernel_gbs": round(copy_gbs, 1),
         "cpu_baseline": cpu_baseline(args) if args.cpu_baseline == "auto" else None,
     }

@@ -77,8 +77,8 @@ int validate(const lbm_problem *p) {
     }
     if (p->zgb && p->side[LBM_SIDE_RIGHT].kind != LBM_BC_PRESSURE)
         return fail(LBM_ERR_ARG, "zgb needs a pressure operator on the right side");
-    if (p->storage != LBM_STORE_F64)
-        return fail(LBM_ERR_UNSUPPORTED, "storage format not available in this build");
+    if (p->storage != LBM_STORE_F64 && p->storage != LBM_STORE_F32 && p->storage != LBM_STORE_F16)
+        return fail(LBM_ERR_ARG, "unknown storage format");
     return LBM_OK;
 }
 
@@ -97,11 +97,24 @@ int upload_side(lbm_ctx *c, int s, const lbm_problem *p, const double *p0, const
 
 double *dbuf(lbm_ctx *c, int k) { return static_cast<double *>(c->buf[k]); }
 
+// storage dispatch of the one-step kernels
+hipError_t run_collide(lbm_ctx *c, bool pull, int from, int to) {
+    if (c->storage == LBM_STORE_F64)
+        return launch_collide_f64(c->prob, pull, dbuf(c, from), dbuf(c, to), c->stream);
+    return launch_collide_lp(c->storage, c->prob, pull, c->buf[from], c->buf[to], c->stream);
+}
+
+hipError_t run_stream(lbm_ctx *c, int from, int to) {
+    if (c->storage == LBM_STORE_F64)
+        return launch_stream_f64(c->prob.lay, dbuf(c, from), dbuf(c, to), c->stream);
+    return launch_stream_lp(c->storage, c->prob.lay, c->buf[from], c->buf[to], c->stream);
+}
+
 // Bring the context into the collided convention (P = buf[cur]).
 int to_collided(lbm_ctx *c) {
     if (c->collided) return LBM_OK;
     // one collision of step m from the streamed state: L -> T
-    HIPCHK(launch_collide_f64(c->prob, false, dbuf(c, c->cur), dbuf(c, 1 - c->cur), c->stream));
+    HIPCHK(run_collide(c, false, c->cur, 1 - c->cur));
     c->cur = 1 - c->cur;
     c->collided = 1;
     c->steps += 1;  // the collided state after this represents step m+1
@@ -111,7 +124,7 @@ int to_collided(lbm_ctx *c) {
 // Bring the context into the streamed convention (L = buf[cur]).
 int to_streamed(lbm_ctx *c) {
     if (!c->collided) return LBM_OK;
-    HIPCHK(launch_stream_f64(c->prob.lay, dbuf(c, c->cur), dbuf(c, 1 - c->cur), c->stream));
+    HIPCHK(run_stream(c, c->cur, 1 - c->cur));
     c->cur = 1 - c->cur;
     c->collided = 0;
     return LBM_OK;
@@ -126,10 +139,8 @@ int step_impl(lbm_ctx *c, int variant, int nsteps) {
         int rc = to_streamed(c);
         if (rc) return rc;
         for (int k = 0; k < nsteps; ++k) {
-            HIPCHK(launch_collide_f64(c->prob, false, dbuf(c, c->cur), dbuf(c, 1 - c->cur),
-                                      c->stream));
-            HIPCHK(launch_stream_f64(c->prob.lay, dbuf(c, 1 - c->cur), dbuf(c, c->cur),
-                                     c->stream));
+            HIPCHK(run_collide(c, false, c->cur, 1 - c->cur));
+            HIPCHK(run_stream(c, 1 - c->cur, c->cur));
             c->steps += 1;
         }
         return LBM_OK;
@@ -143,8 +154,7 @@ int step_impl(lbm_ctx *c, int variant, int nsteps) {
             k = 1;
         }
         for (; k < nsteps; ++k) {
-            HIPCHK(launch_collide_f64(c->prob, true, dbuf(c, c->cur), dbuf(c, 1 - c->cur),
-                                      c->stream));
+            HIPCHK(run_collide(c, true, c->cur, 1 - c->cur));
             c->cur = 1 - c->cur;
             c->steps += 1;
         }
@@ -156,6 +166,8 @@ int step_impl(lbm_ctx *c, int variant, int nsteps) {
         // with one fused single step
         if (c->prob.lay.lx < 4)
             return fail(LBM_ERR_UNSUPPORTED, "two-step variants need lx >= 4");
+        if (c->storage != LBM_STORE_F64)
+            return fail(LBM_ERR_UNSUPPORTED, "two-step variants are built for FP64 storage only");
         int k = 0;
         if (!c->collided) {
             int rc = to_collided(c);
@@ -177,8 +189,7 @@ int step_impl(lbm_ctx *c, int variant, int nsteps) {
             c->steps += 2;
         }
         if (k < nsteps) {
-            HIPCHK(launch_collide_f64(c->prob, true, dbuf(c, c->cur), dbuf(c, 1 - c->cur),
-                                      c->stream));
+            HIPCHK(run_collide(c, true, c->cur, 1 - c->cur));
             c->cur = 1 - c->cur;
             c->steps += 1;
         }
@@ -190,7 +201,7 @@ int step_impl(lbm_ctx *c, int variant, int nsteps) {
 }
 
 // host node layout <-> device planes, interior nodes only
-int transfer(lbm_ctx *c, double *host, int k, bool to_device) {
+int transfer_f64(lbm_ctx *c, double *host, double *devbuf, bool to_device) {
     const Layout &L = c->prob.lay;
     size_t plane_bytes = (size_t)L.plane * sizeof(double);
     double *stage = nullptr;
@@ -198,7 +209,7 @@ int transfer(lbm_ctx *c, double *host, int k, bool to_device) {
     int ny = L.ly + 2;
     hipError_t e = hipSuccess;
     for (int i = 0; i < 9 && e == hipSuccess; ++i) {
-        double *dev = dbuf(c, k) + (int64_t)i * L.plane;
+        double *dev = devbuf + (int64_t)i * L.plane;
         if (to_device) {
             e = hipMemcpy(stage, dev, plane_bytes, hipMemcpyDeviceToHost);  // keep ghosts
             if (e != hipSuccess) break;
@@ -218,6 +229,24 @@ int transfer(lbm_ctx *c, double *host, int k, bool to_device) {
     if (e != hipSuccess) return hip_fail(e, "population transfer");
     if (e2 != hipSuccess) return hip_fail(e2, "hipHostFree");
     return LBM_OK;
+}
+
+// host node layout (FP64) <-> device buffer k of any storage
+int transfer(lbm_ctx *c, double *host, int k, bool to_device) {
+    if (c->storage == LBM_STORE_F64) return transfer_f64(c, host, dbuf(c, k), to_device);
+    const Layout &L = c->prob.lay;
+    double *tmp = nullptr;
+    HIPCHK(hipMalloc((void **)&tmp, 9 * (size_t)L.plane * sizeof(double)));
+    hipError_t e = launch_widen(c->storage, L.plane, c->buf[k], tmp, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    int rc = e == hipSuccess ? transfer_f64(c, host, tmp, to_device) : hip_fail(e, "widen");
+    if (!rc && to_device) {
+        e = launch_narrow(c->storage, L.plane, tmp, c->buf[k], c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "narrow");
+    }
+    (void)hipFree(tmp);
+    return rc;
 }
 
 }  // namespace
@@ -324,7 +353,9 @@ int lbm_hip_init_equilibrium(lbm_ctx *c, const double *init_ux) {
     hipError_t e = hipMemcpy(d_ux, init_ux, (L.ly + 2) * sizeof(double), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemsetAsync(c->buf[0], 0, bytes, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->buf[1], 0, bytes, c->stream);
-    if (e == hipSuccess) e = launch_init_f64(L, d_ux, dbuf(c, 0), c->stream);
+    if (e == hipSuccess)
+        e = c->storage == LBM_STORE_F64 ? launch_init_f64(L, d_ux, dbuf(c, 0), c->stream)
+                                        : launch_init_lp(c->storage, L, d_ux, c->buf[0], c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(d_ux);
     if (e != hipSuccess) return hip_fail(e, "init_equilibrium");
@@ -359,7 +390,7 @@ int lbm_hip_download(lbm_ctx *c, double *lattice, double *tmp, double *outlet_rh
     int lat_buf = c->cur, tmp_buf = 1 - c->cur;
     if (c->collided) {
         // post-stream state into the free buffer; P itself is left as it is
-        HIPCHK(launch_stream_f64(c->prob.lay, dbuf(c, c->cur), dbuf(c, 1 - c->cur), c->stream));
+        HIPCHK(run_stream(c, c->cur, 1 - c->cur));
         lat_buf = 1 - c->cur;
         tmp_buf = c->cur;
     }
@@ -414,8 +445,11 @@ int lbm_hip_macros(lbm_ctx *c, double *rho, double *ux, double *uy) {
     size_t n = (size_t)L.lx * L.ly;
     double *d = nullptr;
     HIPCHK(hipMalloc((void **)&d, 3 * n * sizeof(double)));
-    hipError_t e = launch_macros_f64(L, c->collided, dbuf(c, c->cur), d, d + n, d + 2 * n,
-                                     c->stream);
+    hipError_t e = c->storage == LBM_STORE_F64
+                       ? launch_macros_f64(L, c->collided, dbuf(c, c->cur), d, d + n, d + 2 * n,
+                                           c->stream)
+                       : launch_macros_lp(c->storage, L, c->collided, c->buf[c->cur], d, d + n,
+                                          d + 2 * n, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e == hipSuccess && rho) e = hipMemcpy(rho, d, n * sizeof(double), hipMemcpyDeviceToHost);
     if (e == hipSuccess && ux) e = hipMemcpy(ux, d + n, n * sizeof(double), hipMemcpyDeviceToHost);
@@ -428,8 +462,14 @@ int lbm_hip_macros(lbm_ctx *c, double *rho, double *ux, double *uy) {
 
 int lbm_hip_diagnostics(lbm_ctx *c, double out[4]) {
     if (!c || !out) return fail(LBM_ERR_ARG, "null argument");
-    HIPCHK(launch_diag_f64(c->prob, c->collided, dbuf(c, c->cur), c->d_partial, c->d_out4,
-                           c->stream));
+    if (c->storage == LBM_STORE_F64) {
+        HIPCHK(launch_diag_f64(c->prob, c->collided, dbuf(c, c->cur), c->d_partial, c->d_out4,
+                               c->stream));
+    } else {
+        HIPCHK(launch_diag_lp(c->storage, c->prob, c->collided, c->buf[c->cur], c->d_partial,
+                              c->stream));
+        HIPCHK(launch_diag_final(c->d_partial, c->npartial, c->d_out4, c->stream));
+    }
     HIPCHK(hipMemcpyAsync(out, c->d_out4, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return LBM_OK;

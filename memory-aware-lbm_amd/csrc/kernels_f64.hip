@@ -26,6 +26,7 @@
 //  k_macros / k_diag moments and reductions of the post-stream state.
 #include "launch.hpp"
 #include "collide_f64.hpp"
+#include "fields.hpp"
 
 namespace lbm {
 
@@ -118,81 +119,6 @@ __global__ void k_init(Layout L, const double *__restrict__ init_ux, double *__r
     for_pops([&]<int I>() { dst[I * L.plane + n] = feq<I>(1., u, 0., usqr); });
 }
 
-template <bool PULL>
-__global__ void k_macros(Layout L, const double *__restrict__ src, double *rho, double *ux,
-                         double *uy) {
-    int y = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    int x = blockIdx.y + 1;
-    if (y > L.ly) return;
-    int64_t n = L.node(x, y);
-    double f[9];
-    for_pops([&]<int I>() {
-        f[I] = PULL ? src[I * L.plane + n - cx(I) * L.pitch - cy(I)] : src[I * L.plane + n];
-    });
-    double r, u, v;
-    macros(f, r, u, v);
-    int64_t o = (int64_t)(x - 1) * L.ly + (y - 1);
-    if (rho) rho[o] = r;
-    if (ux) ux[o] = u;
-    if (uy) uy[o] = v;
-}
-
-// ---- diagnostics: block partial sums, then one ordered final sum --------
-
-__device__ __forceinline__ bool solid_node(const Problem &p, int x, int y) {
-    return p.r2 >= 0 && side_of(p, x, y) < 0 && in_disc(p, x, y);
-}
-
-template <bool PULL>
-__global__ __launch_bounds__(256) void k_diag(Problem p, const double *__restrict__ src,
-                                              double *__restrict__ partial) {
-    const Layout &L = p.lay;
-    int y = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    int x = blockIdx.y + 1;
-    double acc[4] = {0., 0., 0., 0.};
-    if (y <= L.ly) {
-        int64_t n = L.node(x, y);
-        double f[9];
-        for_pops([&]<int I>() {
-            f[I] = PULL ? src[I * L.plane + n - cx(I) * L.pitch - cy(I)] : src[I * L.plane + n];
-        });
-        bool solid = solid_node(p, x, y);
-        if (!solid) {
-            double r, u, v;
-            macros(f, r, u, v);
-            acc[2] = 0.5 * r * (u * u + v * v);
-            acc[3] = r;
-        }
-        // momentum exchange across solid / non-solid links
-        for_pops([&]<int I>() {
-            if constexpr (I != 0) {
-                int sx = x - cx(I), sy = y - cy(I);
-                if (!outside(L, sx, sy)) {
-                    bool from_solid = solid_node(p, sx, sy);
-                    if (solid && !from_solid) { acc[0] += cx(I) * f[I]; acc[1] += cy(I) * f[I]; }
-                    if (!solid && from_solid) { acc[0] -= cx(I) * f[I]; acc[1] -= cy(I) * f[I]; }
-                }
-            }
-        });
-    }
-    // wave64 tree reduction, then across the block's waves in LDS
-    __shared__ double red[4][4];
-    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int k = 0; k < 4; ++k) {
-        double v = acc[k];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-        if (lane == 0) red[wave][k] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x < 4) {
-        int nw = (blockDim.x + 63) >> 6;
-        double v = 0.;
-        for (int w = 0; w < nw; ++w) v += red[w][threadIdx.x];
-        int64_t b = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
-        partial[b * 4 + threadIdx.x] = v;
-    }
-}
-
 __global__ void k_diag_final(const double *__restrict__ partial, int64_t nblocks,
                              double *__restrict__ out) {
     // one wave per quantity; lane-strided partial sums then a tree
@@ -274,8 +200,8 @@ hipError_t launch_macros_f64(const Layout &L, bool pull, const double *src, doub
                              double *ux, double *uy, hipStream_t st) {
     int t;
     dim3 g = node_grid(L, t);
-    if (pull) k_macros<true><<<g, t, 0, st>>>(L, src, rho, ux, uy);
-    else k_macros<false><<<g, t, 0, st>>>(L, src, rho, ux, uy);
+    if (pull) k_macros<double, true><<<g, t, 0, st>>>(L, src, rho, ux, uy);
+    else k_macros<double, false><<<g, t, 0, st>>>(L, src, rho, ux, uy);
     return hipGetLastError();
 }
 
@@ -289,8 +215,8 @@ hipError_t launch_diag_f64(const Problem &p, bool pull, const double *src, doubl
                            double *out, hipStream_t st) {
     int t;
     dim3 g = node_grid(p.lay, t);
-    if (pull) k_diag<true><<<g, t, 0, st>>>(p, src, partial);
-    else k_diag<false><<<g, t, 0, st>>>(p, src, partial);
+    if (pull) k_diag<double, true><<<g, t, 0, st>>>(p, src, partial);
+    else k_diag<double, false><<<g, t, 0, st>>>(p, src, partial);
     k_diag_final<<<1, 256, 0, st>>>(partial, (int64_t)g.x * g.y, out);
     return hipGetLastError();
 }
@@ -300,6 +226,11 @@ hipError_t launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t st
     int blocks = (int)((n + 255) / 256 < 256 * 1024 ? (n + 255) / 256 : 256 * 1024);
     if (blocks < 1) blocks = 1;
     k_copy<<<blocks, 256, 0, st>>>((const dbl2 *)src, (dbl2 *)dst, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_diag_final(const double *partial, int64_t nblocks, double *out, hipStream_t st) {
+    k_diag_final<<<1, 256, 0, st>>>(partial, nblocks, out);
     return hipGetLastError();
 }
 

@@ -1,0 +1,244 @@
+// kernels_lowp.hip -- reduced-precision storage: FP32 or FP16 populations.
+//
+// What is stored is the deviation g_i = f_i - w_i from the rest-state
+// weights: it is O(u) instead of O(1), so the storage mantissa is spent on the
+// flow rather than on the constant part.  Bulk collisions run in FP32 on the
+// deviations:
+//   rho = 1 + sum g,  j = sum c g,  u = j / rho,
+//   geq_i = w_i (drho + rho (3 c.u + 9/2 (c.u)^2 - 3/2 u^2)),
+//   g_i  <- g_i + omega (geq_i - g_i).
+// Perimeter nodes widen to FP64 (f = w + g) and run the FP64 wall operators of
+// the bit-exact path, including the outlet densities of the zero-gradient
+// extrapolation; bounce-back is a permutation.  Moments and reductions are
+// accumulated in FP64.  The state conventions (streamed / collided, the ghost
+// ring carrying the unknown populations) are those of kernels_f64.hip.
+//
+// No counterpart in the reference (its arithmetic is FP64 throughout); the
+// error against the FP64 path is measured, not assumed (tests/).
+#include "launch.hpp"
+#include "collide_f64.hpp"
+#include "fields.hpp"
+
+namespace lbm {
+
+template <typename S> __device__ __forceinline__ float ld(const S *p) { return (float)*p; }
+template <typename S> __device__ __forceinline__ void st_nt(S v, S *p) {
+    __builtin_nontemporal_store(v, p);
+}
+
+template <int I> __device__ __forceinline__ float cdotu_f(float ux, float uy) {
+    if constexpr (I == 0) return 0.f;
+    else if constexpr (I == 1) return ux;
+    else if constexpr (I == 2) return uy;
+    else if constexpr (I == 3) return -ux;
+    else if constexpr (I == 4) return -uy;
+    else if constexpr (I == 5) return ux + uy;
+    else if constexpr (I == 6) return uy - ux;
+    else if constexpr (I == 7) return -ux - uy;
+    else return ux - uy;
+}
+
+// BGK on deviations, FP32
+__device__ __forceinline__ void bgk_dev(float g[9], float omega) {
+    float north = g[2] + g[5] + g[6];
+    float mid = g[0] + g[1] + g[3];
+    float south = g[4] + g[7] + g[8];
+    float drho = north + mid + south;
+    float rho = 1.f + drho;
+    float jx = (g[1] + g[5] + g[8]) - (g[3] + g[6] + g[7]);
+    float jy = north - south;
+    float inv = 1.f / rho;
+    float ux = jx * inv, uy = jy * inv;
+    float usq = ux * ux + uy * uy;
+    for_pops([&]<int I>() {
+        float cu = cdotu_f<I>(ux, uy);
+        float geq = (float)wgt(I) * (drho + rho * (3.f * cu + 4.5f * cu * cu - 1.5f * usq));
+        g[I] += omega * (geq - g[I]);
+    });
+}
+
+__device__ __forceinline__ void bounce_dev(float g[9]) {
+    float t;
+    t = g[1]; g[1] = g[3]; g[3] = t;
+    t = g[2]; g[2] = g[4]; g[4] = t;
+    t = g[5]; g[5] = g[7]; g[7] = t;
+    t = g[6]; g[6] = g[8]; g[8] = t;
+}
+
+template <typename S, bool PULL>
+__device__ __forceinline__ double density_lp(const Layout &L, const S *__restrict__ s, int x,
+                                             int y) {
+    double f[9];
+    int64_t n = L.node(x, y);
+    for_pops([&]<int I>() {
+        float g = PULL ? ld(&s[I * L.plane + n - cx(I) * L.pitch - cy(I)]) : ld(&s[I * L.plane + n]);
+        f[I] = wgt(I) + (double)g;
+    });
+    return rho_of(f);
+}
+
+template <typename S, bool PULL>
+__device__ __forceinline__ void perimeter_lp(const Problem &p, int s, int x, int y, float g[9],
+                                          const S *__restrict__ src, S *__restrict__ dst) {
+    const Layout &L = p.lay;
+    int64_t n = L.node(x, y);
+    for_pops([&]<int I>() {
+        if constexpr (I != 0) {
+            int gx = x - cx(I), gy = y - cy(I);
+            if (outside(L, gx, gy)) dst[I * L.plane + L.node(gx, gy)] = PULL ? src[I * L.plane + n]
+                                                                             : dst[I * L.plane + n];
+        }
+    });
+    double f[9];
+    for_pops([&]<int I>() { f[I] = wgt(I) + (double)g[I]; });
+    double rp0 = 0.;
+    if (s == RIGHT && p.zgb)
+        rp0 = zgb_rho(density_lp<S, PULL>(L, src, L.lx - 1, y), density_lp<S, PULL>(L, src, L.lx - 2, y));
+    if (s == RIGHT) p.outlet_rho[y] = p.zgb ? rp0 : p.side[RIGHT].p0[y];
+    collide_class(p, s, x, y, f, rp0);
+    for_pops([&]<int I>() { g[I] = (float)(f[I] - wgt(I)); });
+}
+
+template <typename S, bool PULL>
+__global__ __launch_bounds__(256) void k_collide_lp(Problem p, const S *__restrict__ src,
+                                                    S *__restrict__ dst) {
+    const Layout &L = p.lay;
+    int y = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int x = blockIdx.y + 1;
+    if (y > L.ly) return;
+    int64_t n = L.node(x, y);
+    float g[9];
+    for_pops([&]<int I>() {
+        g[I] = PULL ? ld(&src[I * L.plane + n - cx(I) * L.pitch - cy(I)]) : ld(&src[I * L.plane + n]);
+    });
+    int s = side_of(p, x, y);
+    if (s >= 0) perimeter_lp<S, PULL>(p, s, x, y, g, src, dst);
+    else if (p.r2 >= 0 && in_disc(p, x, y)) bounce_dev(g);
+    else bgk_dev(g, (float)p.omega);
+    for_pops([&]<int I>() { st_nt((S)g[I], &dst[I * L.plane + n]); });
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void k_stream_lp(Layout L, const S *__restrict__ src,
+                                                   S *__restrict__ dst) {
+    int y = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int x = blockIdx.y + 1;
+    if (y > L.ly) return;
+    int64_t n = L.node(x, y);
+    for_pops([&]<int I>() { dst[I * L.plane + n] = src[I * L.plane + n - cx(I) * L.pitch - cy(I)]; });
+}
+
+template <typename S>
+__global__ void k_init_lp(Layout L, const double *__restrict__ init_ux, S *__restrict__ dst) {
+    int y = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int x = blockIdx.y + 1;
+    if (y > L.ly) return;
+    double u = init_ux[y];
+    double usqr = u * u;
+    int64_t n = L.node(x, y);
+    for_pops([&]<int I>() { dst[I * L.plane + n] = (S)(feq<I>(1., u, 0., usqr) - wgt(I)); });
+}
+
+// widen a low-precision buffer to FP64 populations (f = w + g), whole planes
+template <typename S>
+__global__ void k_widen(int64_t plane, const S *__restrict__ src, double *__restrict__ dst) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= plane) return;
+    int i = blockIdx.y;
+    double w = i == 0 ? 4. / 9. : (i <= 4 ? 1. / 9. : 1. / 36.);
+    dst[(int64_t)i * plane + k] = w + (double)(float)src[(int64_t)i * plane + k];
+}
+
+// narrow FP64 populations to stored deviations
+template <typename S>
+__global__ void k_narrow(int64_t plane, const double *__restrict__ src, S *__restrict__ dst) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= plane) return;
+    int i = blockIdx.y;
+    double w = i == 0 ? 4. / 9. : (i <= 4 ? 1. / 9. : 1. / 36.);
+    dst[(int64_t)i * plane + k] = (S)(src[(int64_t)i * plane + k] - w);
+}
+
+// ---- launchers ----------------------------------------------------------
+
+static dim3 grid_lp(const Layout &L, int &threads) {
+    threads = L.ly >= 256 ? 256 : ((L.ly + 63) / 64) * 64;
+    return dim3((L.ly + threads - 1) / threads, L.lx);
+}
+
+template <typename S>
+static hipError_t collide_t(const Problem &p, bool pull, const void *src, void *dst, hipStream_t st) {
+    int t;
+    dim3 g = grid_lp(p.lay, t);
+    if (pull) k_collide_lp<S, true><<<g, t, 0, st>>>(p, (const S *)src, (S *)dst);
+    else k_collide_lp<S, false><<<g, t, 0, st>>>(p, (const S *)src, (S *)dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_collide_lp(int storage, const Problem &p, bool pull, const void *src, void *dst,
+                             hipStream_t st) {
+    return storage == 1 ? collide_t<float>(p, pull, src, dst, st)
+                        : collide_t<_Float16>(p, pull, src, dst, st);
+}
+
+hipError_t launch_stream_lp(int storage, const Layout &L, const void *src, void *dst,
+                            hipStream_t st) {
+    int t;
+    dim3 g = grid_lp(L, t);
+    if (storage == 1) k_stream_lp<float><<<g, t, 0, st>>>(L, (const float *)src, (float *)dst);
+    else k_stream_lp<_Float16><<<g, t, 0, st>>>(L, (const _Float16 *)src, (_Float16 *)dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_lp(int storage, const Layout &L, const double *init_ux, void *dst,
+                          hipStream_t st) {
+    int t;
+    dim3 g = grid_lp(L, t);
+    if (storage == 1) k_init_lp<float><<<g, t, 0, st>>>(L, init_ux, (float *)dst);
+    else k_init_lp<_Float16><<<g, t, 0, st>>>(L, init_ux, (_Float16 *)dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_macros_lp(int storage, const Layout &L, bool pull, const void *src, double *rho,
+                            double *ux, double *uy, hipStream_t st) {
+    int t;
+    dim3 g = grid_lp(L, t);
+    if (storage == 1) {
+        if (pull) k_macros<float, true><<<g, t, 0, st>>>(L, (const float *)src, rho, ux, uy);
+        else k_macros<float, false><<<g, t, 0, st>>>(L, (const float *)src, rho, ux, uy);
+    } else {
+        if (pull) k_macros<_Float16, true><<<g, t, 0, st>>>(L, (const _Float16 *)src, rho, ux, uy);
+        else k_macros<_Float16, false><<<g, t, 0, st>>>(L, (const _Float16 *)src, rho, ux, uy);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_diag_lp(int storage, const Problem &p, bool pull, const void *src,
+                          double *partial, hipStream_t st) {
+    int t;
+    dim3 g = grid_lp(p.lay, t);
+    if (storage == 1) {
+        if (pull) k_diag<float, true><<<g, t, 0, st>>>(p, (const float *)src, partial);
+        else k_diag<float, false><<<g, t, 0, st>>>(p, (const float *)src, partial);
+    } else {
+        if (pull) k_diag<_Float16, true><<<g, t, 0, st>>>(p, (const _Float16 *)src, partial);
+        else k_diag<_Float16, false><<<g, t, 0, st>>>(p, (const _Float16 *)src, partial);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_widen(int storage, int64_t plane, const void *src, double *dst, hipStream_t st) {
+    dim3 g((unsigned)((plane + 255) / 256), 9);
+    if (storage == 1) k_widen<float><<<g, 256, 0, st>>>(plane, (const float *)src, dst);
+    else k_widen<_Float16><<<g, 256, 0, st>>>(plane, (const _Float16 *)src, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_narrow(int storage, int64_t plane, const double *src, void *dst, hipStream_t st) {
+    dim3 g((unsigned)((plane + 255) / 256), 9);
+    if (storage == 1) k_narrow<float><<<g, 256, 0, st>>>(plane, src, (float *)dst);
+    else k_narrow<_Float16><<<g, 256, 0, st>>>(plane, src, (_Float16 *)dst);
+    return hipGetLastError();
+}
+
+}  // namespace lbm

@@ -21,6 +21,22 @@ hipError_t launch_diag_f64(const Problem &p, bool pull, const double *src, doubl
 hipError_t launch_node_op_f64(int which, int n, double *f, const double *p0, const double *p1,
                               double omega, hipStream_t st);
 
+hipError_t launch_diag_final(const double *partial, int64_t nblocks, double *out, hipStream_t st);
+
+// Reduced-precision storage (FP32 = 1, FP16 = 2): kernels_lowp.hip
+hipError_t launch_collide_lp(int storage, const Problem &p, bool pull, const void *src, void *dst,
+                             hipStream_t st);
+hipError_t launch_stream_lp(int storage, const Layout &L, const void *src, void *dst,
+                            hipStream_t st);
+hipError_t launch_init_lp(int storage, const Layout &L, const double *init_ux, void *dst,
+                          hipStream_t st);
+hipError_t launch_macros_lp(int storage, const Layout &L, bool pull, const void *src, double *rho,
+                            double *ux, double *uy, hipStream_t st);
+hipError_t launch_diag_lp(int storage, const Problem &p, bool pull, const void *src,
+                          double *partial, hipStream_t st);
+hipError_t launch_widen(int storage, int64_t plane, const void *src, double *dst, hipStream_t st);
+hipError_t launch_narrow(int storage, int64_t plane, const double *src, void *dst, hipStream_t st);
+
 // Two-step (temporally blocked) kernels: kernels_2step_f64.hip
 struct PanelGrid {
     int ty;   // rows owned per panel (<= 254)

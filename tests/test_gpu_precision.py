@@ -1,0 +1,87 @@
+"""Reduced-precision storage (FP32 / FP16 deviations f_i - w_i, FP32 collision
+math, FP64 wall operators and reductions) against the bit-exact FP64 path.
+No reference counterpart: the bounds are the north-star's (FP32 velocity
+error <= 1e-5 relative) and the measured FP16 behaviour."""
+import numpy as np
+import pytest
+
+import lbm_amd as L
+
+pytestmark = pytest.mark.gpu
+
+
+def run(ch, steps, storage, variant="combine"):
+    with L.Lattice(ch, storage=storage) as lat:
+        lat.init_equilibrium()
+        lat.step(steps, variant)
+        return lat.macros(), lat.diagnostics()
+
+
+def rel_l2(a, b):
+    return float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
+
+
+@pytest.fixture(scope="module")
+def c1_fields():
+    ch = L.Channel.geometry(256, 128, "B")
+    out = {s: run(ch, 2000, s) for s in ("f64", "f32", "f16")}
+    return ch, out
+
+
+def test_f32_velocity_error(c1_fields):
+    ch, out = c1_fields
+    (r64, u64, v64), _ = out["f64"]
+    (r32, u32, v32), _ = out["f32"]
+    vel64 = np.stack([u64, v64]); vel32 = np.stack([u32, v32])
+    err = rel_l2(vel32, vel64)
+    assert err <= 1e-5, err
+    assert np.abs(r32 - r64).max() <= 1e-6
+
+
+def test_f16_velocity_error(c1_fields):
+    ch, out = c1_fields
+    (r64, u64, v64), _ = out["f64"]
+    (r16, u16, v16), _ = out["f16"]
+    err = rel_l2(np.stack([u16, v16]), np.stack([u64, v64]))
+    assert np.isfinite(err) and err <= 2e-2, err
+
+
+def test_reduced_drag_lift_close(c1_fields):
+    ch, out = c1_fields
+    d64 = out["f64"][1]
+    d32 = out["f32"][1]
+    assert abs(d32[0] - d64[0]) <= 1e-4 * abs(d64[0]) + 1e-9
+    assert abs(d32[3] - d64[3]) <= 1e-6 * d64[3]
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_reduced_variants_identical(storage):
+    """origin (collide + stream kernels) and combine (fused pull) do the same
+    arithmetic on the same stored values, so they agree bit for bit."""
+    ch = L.Channel.geometry(96, 48, "A")
+    with L.Lattice(ch, storage=storage) as a, L.Lattice(ch, storage=storage) as b:
+        a.init_equilibrium(); b.init_equilibrium()
+        a.step(37, "origin"); b.step(37, "combine")
+        la, ta, pa = a.download(); lb, tb, pb = b.download()
+    assert np.array_equal(la, lb) and np.array_equal(ta, tb) and np.array_equal(pa, pb)
+
+
+@pytest.mark.parametrize("storage,tol", [("f32", 1e-7), ("f16", 1e-3)])
+def test_reduced_upload_download_round_trip(orc, storage, tol):
+    from oracle_lib import Channel as OC
+    ch = OC.geometry(64, 32, "B")
+    _, lref, tref, pref = orc.run(ch, 11)
+    dch = L.Channel(ch.lx, ch.ly, ch.obst_x, ch.obst_y, ch.obst_r)
+    with L.Lattice(dch, storage=storage) as lat:
+        lat.upload(lref, tref, pref)
+        lo, to, po = lat.download()
+    assert np.abs(lo[1:-1, 1:-1] - lref[1:-1, 1:-1]).max() <= tol
+    assert np.abs(to[1:-1, 1:-1] - tref[1:-1, 1:-1]).max() <= tol
+
+
+def test_two_step_reduced_is_refused():
+    ch = L.Channel.geometry(64, 32, "B")
+    with L.Lattice(ch, storage="f32") as lat:
+        lat.init_equilibrium()
+        with pytest.raises(L.LbmError):
+            lat.step(4, "tight_block")
