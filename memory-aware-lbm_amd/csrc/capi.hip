@@ -295,8 +295,11 @@ int lbm_hip_create(const lbm_problem *p, lbm_ctx **out) {
     for (int k = 0; k < 2; ++k) {
         e = hipMalloc(&c->buf[k], bytes);
         if (e != hipSuccess) return cleanup(e, "hipMalloc populations");
-        e = hipMemset(c->buf[k], 0, bytes);
-        if (e != hipSuccess) return cleanup(e, "hipMemset populations");
+        e = p->storage == LBM_STORE_F64
+                ? hipMemset(c->buf[k], 0, bytes)
+                : launch_fill_zero_f(p->storage, c->prob.lay.plane, c->buf[k], nullptr);
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+        if (e != hipSuccess) return cleanup(e, "clearing populations");
     }
     for (int s = 0; s < 4; ++s) {
         int n = side_len(p, s);
@@ -351,8 +354,12 @@ int lbm_hip_init_equilibrium(lbm_ctx *c, const double *init_ux) {
     double *d_ux = nullptr;
     HIPCHK(hipMalloc((void **)&d_ux, (L.ly + 2) * sizeof(double)));
     hipError_t e = hipMemcpy(d_ux, init_ux, (L.ly + 2) * sizeof(double), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemsetAsync(c->buf[0], 0, bytes, c->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(c->buf[1], 0, bytes, c->stream);
+    // both buffers start as f = 0 (calloc, lb.c:89-92); in deviation storage
+    // that is g = -w, not a zero bit pattern
+    for (int k = 0; k < 2 && e == hipSuccess; ++k)
+        e = c->storage == LBM_STORE_F64 ? hipMemsetAsync(c->buf[k], 0, bytes, c->stream)
+                                        : launch_fill_zero_f(c->storage, L.plane, c->buf[k],
+                                                             c->stream);
     if (e == hipSuccess)
         e = c->storage == LBM_STORE_F64 ? launch_init_f64(L, d_ux, dbuf(c, 0), c->stream)
                                         : launch_init_lp(c->storage, L, d_ux, c->buf[0], c->stream);

@@ -38,23 +38,34 @@ template <int I> __device__ __forceinline__ float cdotu_f(float ux, float uy) {
     else return ux - uy;
 }
 
-// BGK on deviations, FP32
+// BGK on deviations, FP32.  rho = 1 + drho is never formed in FP32 (it would
+// drop the low bits of drho every step): u = j / (1 + drho) is evaluated as
+// j - j drho / (1 + drho) and rho X as X + drho X, so the rounding of 1 + drho
+// only touches terms that are already O(drho) small.
 __device__ __forceinline__ void bgk_dev(float g[9], float omega) {
     float north = g[2] + g[5] + g[6];
     float mid = g[0] + g[1] + g[3];
     float south = g[4] + g[7] + g[8];
     float drho = north + mid + south;
-    float rho = 1.f + drho;
     float jx = (g[1] + g[5] + g[8]) - (g[3] + g[6] + g[7]);
     float jy = north - south;
-    float inv = 1.f / rho;
-    float ux = jx * inv, uy = jy * inv;
+    float shrink = drho / (1.f + drho);
+    float ux = jx - jx * shrink, uy = jy - jy * shrink;
     float usq = ux * ux + uy * uy;
+    // moving populations first; the rest population's equilibrium closes the
+    // mass balance exactly (sum geq = drho), so FP32 rounding of the
+    // equilibrium cannot bias the density step after step
+    float rest = drho;
     for_pops([&]<int I>() {
-        float cu = cdotu_f<I>(ux, uy);
-        float geq = (float)wgt(I) * (drho + rho * (3.f * cu + 4.5f * cu * cu - 1.5f * usq));
-        g[I] += omega * (geq - g[I]);
+        if constexpr (I != 0) {
+            float cu = cdotu_f<I>(ux, uy);
+            float X = 3.f * cu + 4.5f * cu * cu - 1.5f * usq;
+            float geq = (float)wgt(I) * (drho + (X + drho * X));
+            rest -= geq;
+            g[I] += omega * (geq - g[I]);
+        }
     });
+    g[0] += omega * (rest - g[0]);
 }
 
 __device__ __forceinline__ void bounce_dev(float g[9]) {
@@ -137,6 +148,17 @@ __global__ void k_init_lp(Layout L, const double *__restrict__ init_ux, S *__res
     double usqr = u * u;
     int64_t n = L.node(x, y);
     for_pops([&]<int I>() { dst[I * L.plane + n] = (S)(feq<I>(1., u, 0., usqr) - wgt(I)); });
+}
+
+// f = 0 everywhere (the reference's calloc'd buffers) in deviation storage:
+// g_i = -w_i on every element of the plane, ghost ring included
+template <typename S>
+__global__ void k_fill_zero_f(int64_t plane, S *__restrict__ dst) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= plane) return;
+    int i = blockIdx.y;
+    double w = i == 0 ? 4. / 9. : (i <= 4 ? 1. / 9. : 1. / 36.);
+    dst[(int64_t)i * plane + k] = (S)(-w);
 }
 
 // widen a low-precision buffer to FP64 populations (f = w + g), whole planes
@@ -224,6 +246,13 @@ hipError_t launch_diag_lp(int storage, const Problem &p, bool pull, const void *
         if (pull) k_diag<_Float16, true><<<g, t, 0, st>>>(p, (const _Float16 *)src, partial);
         else k_diag<_Float16, false><<<g, t, 0, st>>>(p, (const _Float16 *)src, partial);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_zero_f(int storage, int64_t plane, void *dst, hipStream_t st) {
+    dim3 g((unsigned)((plane + 255) / 256), 9);
+    if (storage == 1) k_fill_zero_f<float><<<g, 256, 0, st>>>(plane, (float *)dst);
+    else k_fill_zero_f<_Float16><<<g, 256, 0, st>>>(plane, (_Float16 *)dst);
     return hipGetLastError();
 }
 

@@ -34,6 +34,7 @@ hipError_t launch_macros_lp(int storage, const Layout &L, bool pull, const void 
                             double *ux, double *uy, hipStream_t st);
 hipError_t launch_diag_lp(int storage, const Problem &p, bool pull, const void *src,
                           double *partial, hipStream_t st);
+hipError_t launch_fill_zero_f(int storage, int64_t plane, void *dst, hipStream_t st);
 hipError_t launch_widen(int storage, int64_t plane, const void *src, double *dst, hipStream_t st);
 hipError_t launch_narrow(int storage, int64_t plane, const double *src, void *dst, hipStream_t st);
 
