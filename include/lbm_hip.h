@@ -189,6 +189,34 @@ int lbm_hip_apply_node_op(int which, int n, double *f, const double *p0,
  * iters launches; the HBM yardstick the roofline figures are read against. */
 int lbm_hip_copy_bandwidth(size_t bytes, int iters, double *ms_per_copy);
 
+/* ---- batches: many independent channels, one launch per step ----------
+ * No counterpart in the reference (one lattice per process there).  All
+ * channels must use the same storage format.  A step of the batch is one
+ * persistent launch over a table of 256-node tiles of every channel; each
+ * node runs the single-lattice node step, so every channel evolves bit for
+ * bit as it would in its own context (fused pull variant). */
+typedef struct lbm_batch lbm_batch;
+
+int lbm_hip_batch_create(const lbm_problem *probs, int n, lbm_batch **out);
+int lbm_hip_batch_destroy(lbm_batch *b);
+/* init_ux[c] points to ly_c + 2 values (the initial / inlet profile) */
+int lbm_hip_batch_init_equilibrium(lbm_batch *b, const double *const *init_ux);
+/* Launch shape: workgroups per CU of the persistent launch, the cap on the
+ * tile height (32/64/128/256; tiles are 256 nodes), and the number of steps
+ * captured per hipGraph (0 = launch eagerly).  0 leaves a knob unchanged
+ * except graph_steps. */
+int lbm_hip_batch_set_launch(lbm_batch *b, int wgs_per_cu, int ty_cap, int graph_steps);
+int lbm_hip_batch_step(lbm_batch *b, int nsteps);
+int lbm_hip_batch_sync(lbm_batch *b);
+int lbm_hip_batch_time(lbm_batch *b, int nsteps, double *ms);
+/* one channel's state in node layout, as lbm_hip_download */
+int lbm_hip_batch_download(lbm_batch *b, int c, double *lattice, double *tmp,
+                           double *outlet_rho);
+int lbm_hip_batch_macros(lbm_batch *b, int c, double *rho, double *ux, double *uy);
+int64_t lbm_hip_batch_cells(const lbm_batch *b);
+int64_t lbm_hip_batch_steps_done(const lbm_batch *b);
+size_t lbm_hip_batch_footprint(const lbm_batch *b);
+
 /* Description of the last error on this thread. */
 const char *lbm_hip_last_error(void);
 

@@ -1,0 +1,497 @@
+// batch.hip -- many independent channels advanced by one persistent launch
+// per time step.
+//
+// All channels live in two pools (ping-pong), each channel a self-contained
+// SoA lattice with its own ghost ring at an aligned offset.  The work of one
+// step is a flat table of tiles: tile = (channel, ty rows x tx columns) with
+// tx * ty = 256 nodes, ty a power of two >= 32 chosen per channel so short
+// channels do not leave most lanes idle.  A launch has nwg = CUs x
+// wgs_per_cu workgroups; workgroup b walks the contiguous item range
+// [b * per_wg, (b+1) * per_wg), so neighbouring tiles of a channel stay on one
+// XCD's L2.  Each node runs exactly the single-lattice node step
+// (node_step.hpp), so every channel's result is bit-identical to running it
+// alone.  K consecutive steps can be captured into a hipGraph and replayed.
+//
+// No counterpart in the reference (one lattice per process there).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "../../include/lbm_hip.h"
+#include "host_util.hpp"
+#include "launch.hpp"
+#include "node_step.hpp"
+
+namespace lbm {
+
+struct BatchEntry {
+    Problem p;
+    int64_t off;    // element offset of the channel in each pool
+    int64_t item0;  // first work item of the channel
+    int ty, tx, nty;
+};
+
+template <typename S, bool PULL>
+__global__ __launch_bounds__(256) void k_batch(const BatchEntry *__restrict__ ents,
+                                               const int *__restrict__ item_chan, int64_t nitems,
+                                               int64_t per_wg, const S *__restrict__ A,
+                                               S *__restrict__ B) {
+    int64_t it = (int64_t)blockIdx.x * per_wg;
+    int64_t end = it + per_wg < nitems ? it + per_wg : nitems;
+    for (; it < end; ++it) {
+        const int c = __builtin_amdgcn_readfirstlane(item_chan[it]);
+        const BatchEntry &e = ents[c];
+        const int ty = e.ty, nty = e.nty;
+        int64_t local = it - e.item0;
+        int xt = (int)(local / nty);
+        int yt = (int)(local - (int64_t)xt * nty);
+        int x = 1 + xt * e.tx + (int)threadIdx.x / ty;
+        int y = 1 + yt * ty + ((int)threadIdx.x & (ty - 1));
+        if (x > e.p.lay.lx || y > e.p.lay.ly) continue;
+        if constexpr (sizeof(S) == 8)
+            node_step_f64<PULL>(e.p, x, y, (const double *)A + e.off, (double *)B + e.off);
+        else
+            node_step_lp<S, PULL>(e.p, x, y, A + e.off, B + e.off);
+    }
+}
+
+template <typename S>
+static hipError_t batch_launch(bool pull, int nwg, const BatchEntry *ents, const int *item_chan,
+                               int64_t nitems, const void *A, void *B, hipStream_t st) {
+    int64_t per_wg = (nitems + nwg - 1) / nwg;
+    if (pull)
+        k_batch<S, true><<<nwg, 256, 0, st>>>(ents, item_chan, nitems, per_wg, (const S *)A,
+                                             (S *)B);
+    else
+        k_batch<S, false><<<nwg, 256, 0, st>>>(ents, item_chan, nitems, per_wg, (const S *)A,
+                                              (S *)B);
+    return hipGetLastError();
+}
+
+}  // namespace lbm
+
+using namespace lbm;
+
+namespace {
+size_t esize(int storage) { return storage == LBM_STORE_F64 ? 8 : storage == LBM_STORE_F32 ? 4 : 2; }
+}
+
+struct lbm_batch {
+    int n;
+    int storage;
+    std::vector<BatchEntry> ents;
+    BatchEntry *d_ents = nullptr;
+    int *d_item_chan = nullptr;
+    int64_t nitems = 0;
+    void *pool[2] = {nullptr, nullptr};
+    double *params = nullptr;
+    int64_t pool_elems = 0;
+    int64_t cells = 0;
+    int cur = 0;
+    int collided = 0;
+    int64_t steps = 0;
+    hipStream_t stream = nullptr;
+    int num_cu = 256;
+    int wgs_per_cu = 8;
+    int ty_cap = 256;
+    int graph_steps = 0;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};  // by starting buffer
+    std::vector<int> ly_of, lx_of;
+};
+
+namespace {
+
+int ceil_pow2(int v) {
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+void drop_graphs(lbm_batch *b) {
+    for (int k = 0; k < 2; ++k)
+        if (b->graph[k]) {
+            (void)hipGraphExecDestroy(b->graph[k]);
+            b->graph[k] = nullptr;
+        }
+}
+
+// tile shapes and the work table for the current ty_cap
+int build_items(lbm_batch *b) {
+    std::vector<int> chan;
+    int64_t item = 0;
+    for (int c = 0; c < b->n; ++c) {
+        BatchEntry &e = b->ents[c];
+        int ty = std::min(b->ty_cap, std::max(32, ceil_pow2(e.p.lay.ly)));
+        e.ty = ty;
+        e.tx = 256 / ty;
+        e.nty = (e.p.lay.ly + ty - 1) / ty;
+        int ntx = (e.p.lay.lx + e.tx - 1) / e.tx;
+        e.item0 = item;
+        int64_t cnt = (int64_t)ntx * e.nty;
+        chan.insert(chan.end(), (size_t)cnt, c);
+        item += cnt;
+    }
+    if (b->d_item_chan) (void)hipFree(b->d_item_chan);
+    b->d_item_chan = nullptr;
+    b->nitems = item;
+    HIPCHK(hipMalloc((void **)&b->d_item_chan, chan.size() * sizeof(int)));
+    HIPCHK(hipMemcpy(b->d_item_chan, chan.data(), chan.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->d_ents, b->ents.data(), b->ents.size() * sizeof(BatchEntry),
+                     hipMemcpyHostToDevice));
+    drop_graphs(b);
+    return LBM_OK;
+}
+
+hipError_t launch_step(lbm_batch *b, bool pull) {
+    int nwg = std::max(1, b->num_cu * b->wgs_per_cu);
+    const void *src = b->pool[b->cur];
+    void *dst = b->pool[1 - b->cur];
+    switch (b->storage) {
+    case LBM_STORE_F64:
+        return batch_launch<double>(pull, nwg, b->d_ents, b->d_item_chan, b->nitems, src, dst,
+                                    b->stream);
+    case LBM_STORE_F32:
+        return batch_launch<float>(pull, nwg, b->d_ents, b->d_item_chan, b->nitems, src, dst,
+                                   b->stream);
+    default:
+        return batch_launch<_Float16>(pull, nwg, b->d_ents, b->d_item_chan, b->nitems, src, dst,
+                                      b->stream);
+    }
+}
+
+int eager_steps(lbm_batch *b, int nsteps) {
+    for (int k = 0; k < nsteps; ++k) {
+        HIPCHK(launch_step(b, b->collided != 0));
+        b->cur = 1 - b->cur;
+        b->collided = 1;
+        b->steps += 1;
+    }
+    return LBM_OK;
+}
+
+// capture graph_steps pulls starting from buffer `start`
+int capture(lbm_batch *b, int start) {
+    int saved_cur = b->cur;
+    b->cur = start;
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < b->graph_steps && e == hipSuccess; ++k) {
+        e = launch_step(b, true);
+        b->cur = 1 - b->cur;
+    }
+    hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+    b->cur = saved_cur;
+    if (e != hipSuccess) { if (g) (void)hipGraphDestroy(g); return hip_fail(e, "capture launch"); }
+    if (e2 != hipSuccess) return hip_fail(e2, "hipStreamEndCapture");
+    hipError_t e3 = hipGraphInstantiate(&b->graph[start], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e3 != hipSuccess) return hip_fail(e3, "hipGraphInstantiate");
+    return LBM_OK;
+}
+
+int step_impl(lbm_batch *b, int nsteps) {
+    if (nsteps < 0) return fail(LBM_ERR_ARG, "nsteps < 0");
+    if (!b->collided && nsteps > 0) {
+        int rc = eager_steps(b, 1);
+        if (rc) return rc;
+        --nsteps;
+    }
+    if (b->graph_steps > 0) {
+        while (nsteps >= b->graph_steps) {
+            if (!b->graph[b->cur]) {
+                int rc = capture(b, b->cur);
+                if (rc) return rc;
+            }
+            HIPCHK(hipGraphLaunch(b->graph[b->cur], b->stream));
+            if (b->graph_steps & 1) b->cur = 1 - b->cur;
+            b->steps += b->graph_steps;
+            nsteps -= b->graph_steps;
+        }
+    }
+    return eager_steps(b, nsteps);
+}
+
+template <typename T> T *typed(void *p, int64_t off) { return static_cast<T *>(p) + off; }
+
+}  // namespace
+
+extern "C" {
+
+int lbm_hip_batch_create(const lbm_problem *probs, int n, lbm_batch **out) {
+    if (!out || !probs || n < 1) return fail(LBM_ERR_ARG, "bad batch request");
+    *out = nullptr;
+    int storage = probs[0].storage;
+    if (storage < LBM_STORE_F64 || storage > LBM_STORE_F16)
+        return fail(LBM_ERR_ARG, "unknown storage format");
+    for (int c = 0; c < n; ++c) {
+        const lbm_problem &p = probs[c];
+        if (p.storage != storage) return fail(LBM_ERR_ARG, "mixed storage formats in one batch");
+        if (p.lx < 3 || p.ly < 3) return fail(LBM_ERR_ARG, "channel smaller than 3x3");
+        if (!(p.omega > 0.0 && p.omega <= 2.0)) return fail(LBM_ERR_ARG, "omega out of (0, 2]");
+        if (p.zgb && p.side[LBM_SIDE_RIGHT].kind != LBM_BC_PRESSURE)
+            return fail(LBM_ERR_ARG, "zgb needs a pressure operator on the right side");
+    }
+    lbm_batch *b = new (std::nothrow) lbm_batch();
+    if (!b) return fail(LBM_ERR_NOMEM, "host allocation");
+    b->n = n;
+    b->storage = storage;
+    b->ents.resize(n);
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+        b->num_cu = prop.multiProcessorCount;
+    // pool offsets (256-element aligned) and parameter slots
+    int64_t off = 0, poff = 0;
+    std::vector<int64_t> param_off(n);
+    for (int c = 0; c < n; ++c) {
+        const lbm_problem &p = probs[c];
+        BatchEntry &e = b->ents[c];
+        std::memset(&e, 0, sizeof e);
+        e.p.lay = make_layout(p.lx, p.ly);
+        e.p.ox = p.obst_x;
+        e.p.oy = p.obst_y;
+        e.p.r2 = p.obst_r >= 0 ? p.obst_r * p.obst_r : -1;
+        e.p.omega = p.omega;
+        e.p.zgb = p.zgb;
+        e.off = off;
+        off += (9 * e.p.lay.plane + 255) / 256 * 256;
+        param_off[c] = poff;
+        poff += 4 * (int64_t)(p.lx + 2) + 4 * (int64_t)(p.ly + 2) + (p.ly + 2);
+        b->cells += (int64_t)p.lx * p.ly;
+        b->lx_of.push_back(p.lx);
+        b->ly_of.push_back(p.ly);
+    }
+    b->pool_elems = off;
+    auto bail = [&](hipError_t e, const char *what) {
+        lbm_hip_batch_destroy(b);
+        return hip_fail(e, what);
+    };
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return bail(e, "hipStreamCreate");
+    for (int k = 0; k < 2; ++k) {
+        e = hipMalloc(&b->pool[k], (size_t)off * esize(storage));
+        if (e != hipSuccess) return bail(e, "hipMalloc pool");
+    }
+    e = hipMalloc((void **)&b->params, (size_t)poff * sizeof(double));
+    if (e != hipSuccess) return bail(e, "hipMalloc params");
+    e = hipMalloc((void **)&b->d_ents, n * sizeof(BatchEntry));
+    if (e != hipSuccess) return bail(e, "hipMalloc entries");
+    // per-channel side parameters
+    std::vector<double> host(poff, 0.0);
+    for (int c = 0; c < n; ++c) {
+        const lbm_problem &p = probs[c];
+        BatchEntry &ent = b->ents[c];
+        int64_t q = param_off[c];
+        for (int s = 0; s < 4; ++s) {
+            int len = (s == LBM_SIDE_UPPER || s == LBM_SIDE_LOWER) ? p.lx + 2 : p.ly + 2;
+            double *h0 = &host[q], *h1 = &host[q + len];
+            if (p.side[s].p0) std::memcpy(h0, p.side[s].p0, len * sizeof(double));
+            else if (p.side[s].kind == LBM_BC_PRESSURE) std::fill(h0, h0 + len, 1.0);
+            if (p.side[s].p1) std::memcpy(h1, p.side[s].p1, len * sizeof(double));
+            ent.p.side[s].kind = p.side[s].kind;
+            ent.p.side[s].family = p.side[s].family;
+            ent.p.side[s].p0 = b->params + q;
+            ent.p.side[s].p1 = b->params + q + len;
+            if (s == LBM_SIDE_RIGHT) {
+                // outlet density record starts as the right side's p0
+                std::memcpy(&host[param_off[c] + 4 * (int64_t)(p.lx + 2) + 4 * (int64_t)(p.ly + 2)],
+                            h0, len * sizeof(double));
+            }
+            q += 2 * (int64_t)len;
+        }
+        ent.p.outlet_rho = b->params + q;
+    }
+    e = hipMemcpy(b->params, host.data(), (size_t)poff * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return bail(e, "params upload");
+    int rc = build_items(b);
+    if (rc) { std::string m = lbm_g_err; lbm_hip_batch_destroy(b); return fail(rc, m); }
+    *out = b;
+    return LBM_OK;
+}
+
+int lbm_hip_batch_destroy(lbm_batch *b) {
+    if (!b) return LBM_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_graphs(b);
+    for (int k = 0; k < 2; ++k) if (b->pool[k]) (void)hipFree(b->pool[k]);
+    if (b->params) (void)hipFree(b->params);
+    if (b->d_ents) (void)hipFree(b->d_ents);
+    if (b->d_item_chan) (void)hipFree(b->d_item_chan);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return LBM_OK;
+}
+
+int lbm_hip_batch_init_equilibrium(lbm_batch *b, const double *const *init_ux) {
+    if (!b || !init_ux) return fail(LBM_ERR_ARG, "null argument");
+    // both pools to f = 0 (zero bits in FP64, -w in deviation storage)
+    for (int k = 0; k < 2; ++k) {
+        if (b->storage == LBM_STORE_F64) {
+            HIPCHK(hipMemsetAsync(b->pool[k], 0, (size_t)b->pool_elems * 8, b->stream));
+        } else {
+            for (int c = 0; c < b->n; ++c)
+                HIPCHK(launch_fill_zero_f(b->storage, b->ents[c].p.lay.plane,
+                                          (char *)b->pool[k] + b->ents[c].off * esize(b->storage),
+                                          b->stream));
+        }
+    }
+    double *d_ux = nullptr;
+    int64_t tot = 0;
+    for (int c = 0; c < b->n; ++c) tot += b->ly_of[c] + 2;
+    HIPCHK(hipMalloc((void **)&d_ux, tot * sizeof(double)));
+    std::vector<double> h(tot);
+    int64_t q = 0;
+    for (int c = 0; c < b->n; ++c) {
+        std::memcpy(&h[q], init_ux[c], (b->ly_of[c] + 2) * sizeof(double));
+        q += b->ly_of[c] + 2;
+    }
+    hipError_t e = hipMemcpy(d_ux, h.data(), tot * sizeof(double), hipMemcpyHostToDevice);
+    q = 0;
+    for (int c = 0; c < b->n && e == hipSuccess; ++c) {
+        const Layout &L = b->ents[c].p.lay;
+        void *dst = (char *)b->pool[0] + b->ents[c].off * esize(b->storage);
+        e = b->storage == LBM_STORE_F64
+                ? launch_init_f64(L, d_ux + q, (double *)dst, b->stream)
+                : launch_init_lp(b->storage, L, d_ux + q, dst, b->stream);
+        q += b->ly_of[c] + 2;
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    (void)hipFree(d_ux);
+    if (e != hipSuccess) return hip_fail(e, "batch init");
+    b->cur = 0;
+    b->collided = 0;
+    b->steps = 0;
+    return LBM_OK;
+}
+
+int lbm_hip_batch_set_launch(lbm_batch *b, int wgs_per_cu, int ty_cap, int graph_steps) {
+    if (!b || wgs_per_cu < 0 || graph_steps < 0) return fail(LBM_ERR_ARG, "bad launch setting");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if (wgs_per_cu > 0) b->wgs_per_cu = wgs_per_cu;
+    if (graph_steps != b->graph_steps) drop_graphs(b);
+    b->graph_steps = graph_steps;
+    if (ty_cap > 0) {
+        if (ty_cap != 32 && ty_cap != 64 && ty_cap != 128 && ty_cap != 256)
+            return fail(LBM_ERR_ARG, "ty_cap must be 32, 64, 128 or 256");
+        if (ty_cap != b->ty_cap) {
+            b->ty_cap = ty_cap;
+            return build_items(b);
+        }
+    }
+    drop_graphs(b);
+    return LBM_OK;
+}
+
+int lbm_hip_batch_step(lbm_batch *b, int nsteps) {
+    if (!b) return fail(LBM_ERR_ARG, "null batch");
+    return step_impl(b, nsteps);
+}
+
+int lbm_hip_batch_sync(lbm_batch *b) {
+    if (!b) return fail(LBM_ERR_ARG, "null batch");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return LBM_OK;
+}
+
+int lbm_hip_batch_time(lbm_batch *b, int nsteps, double *ms) {
+    if (!b || !ms) return fail(LBM_ERR_ARG, "null argument");
+    if (!b->collided) {
+        int rc = step_impl(b, 1);
+        if (rc) return rc;
+    }
+    hipEvent_t a, z;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&z));
+    hipError_t e = hipEventRecord(a, b->stream);
+    int rc = e == hipSuccess ? step_impl(b, nsteps) : LBM_OK;
+    if (!rc && e == hipSuccess) e = hipEventRecord(z, b->stream);
+    if (!rc && e == hipSuccess) e = hipEventSynchronize(z);
+    float t = 0.f;
+    if (!rc && e == hipSuccess) e = hipEventElapsedTime(&t, a, z);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(z);
+    if (rc) return rc;
+    if (e != hipSuccess) return hip_fail(e, "batch timing");
+    *ms = t;
+    return LBM_OK;
+}
+
+int lbm_hip_batch_download(lbm_batch *b, int c, double *lattice, double *tmp,
+                           double *outlet_rho) {
+    if (!b || c < 0 || c >= b->n) return fail(LBM_ERR_ARG, "bad channel index");
+    const BatchEntry &e = b->ents[c];
+    const Layout &L = e.p.lay;
+    size_t es = esize(b->storage);
+    void *P = (char *)b->pool[b->cur] + e.off * es;
+    void *Q = (char *)b->pool[1 - b->cur] + e.off * es;
+    void *lat_src = P, *tmp_src = Q;
+    if (b->collided) {
+        HIPCHK(b->storage == LBM_STORE_F64
+                   ? launch_stream_f64(L, (const double *)P, (double *)Q, b->stream)
+                   : launch_stream_lp(b->storage, L, P, Q, b->stream));
+        lat_src = Q;
+        tmp_src = P;
+    }
+    HIPCHK(hipStreamSynchronize(b->stream));
+    double *wide = nullptr;
+    if (b->storage != LBM_STORE_F64)
+        HIPCHK(hipMalloc((void **)&wide, 9 * (size_t)L.plane * sizeof(double)));
+    int rc = LBM_OK;
+    void *srcs[2] = {lat_src, tmp_src};
+    double *hosts[2] = {lattice, tmp};
+    for (int k = 0; k < 2 && !rc; ++k) {
+        if (!hosts[k]) continue;
+        double *dev = (double *)srcs[k];
+        if (wide) {
+            hipError_t ee = launch_widen(b->storage, L.plane, srcs[k], wide, b->stream);
+            if (ee == hipSuccess) ee = hipStreamSynchronize(b->stream);
+            if (ee != hipSuccess) { rc = hip_fail(ee, "widen"); break; }
+            dev = wide;
+        }
+        rc = transfer_f64(L, hosts[k], dev, false);
+    }
+    if (wide) (void)hipFree(wide);
+    if (!rc && outlet_rho) {
+        hipError_t ee = hipMemcpy(outlet_rho, e.p.outlet_rho, (L.ly + 2) * sizeof(double),
+                                  hipMemcpyDeviceToHost);
+        if (ee != hipSuccess) rc = hip_fail(ee, "outlet download");
+    }
+    return rc;
+}
+
+int lbm_hip_batch_macros(lbm_batch *b, int c, double *rho, double *ux, double *uy) {
+    if (!b || c < 0 || c >= b->n) return fail(LBM_ERR_ARG, "bad channel index");
+    const BatchEntry &e = b->ents[c];
+    const Layout &L = e.p.lay;
+    size_t n = (size_t)L.lx * L.ly;
+    double *d = nullptr;
+    HIPCHK(hipMalloc((void **)&d, 3 * n * sizeof(double)));
+    void *src = (char *)b->pool[b->cur] + e.off * esize(b->storage);
+    hipError_t er = b->storage == LBM_STORE_F64
+                        ? launch_macros_f64(L, b->collided, (const double *)src, d, d + n,
+                                            d + 2 * n, b->stream)
+                        : launch_macros_lp(b->storage, L, b->collided, src, d, d + n, d + 2 * n,
+                                           b->stream);
+    if (er == hipSuccess) er = hipStreamSynchronize(b->stream);
+    if (er == hipSuccess && rho) er = hipMemcpy(rho, d, n * sizeof(double), hipMemcpyDeviceToHost);
+    if (er == hipSuccess && ux) er = hipMemcpy(ux, d + n, n * sizeof(double), hipMemcpyDeviceToHost);
+    if (er == hipSuccess && uy)
+        er = hipMemcpy(uy, d + 2 * n, n * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (er != hipSuccess) return hip_fail(er, "batch macros");
+    return LBM_OK;
+}
+
+int64_t lbm_hip_batch_cells(const lbm_batch *b) { return b ? b->cells : -1; }
+int64_t lbm_hip_batch_steps_done(const lbm_batch *b) { return b ? b->steps : -1; }
+
+size_t lbm_hip_batch_footprint(const lbm_batch *b) {
+    if (!b) return 0;
+    return 2 * (size_t)b->pool_elems * esize(b->storage) + (size_t)b->nitems * sizeof(int) +
+           b->ents.size() * sizeof(BatchEntry);
+}
+
+}  // extern "C"

@@ -11,31 +11,14 @@
 
 #include "../../include/lbm_hip.h"
 #include "launch.hpp"
+#include "host_util.hpp"
 
 using namespace lbm;
 
+thread_local std::string lbm_g_err;
+
 namespace {
-
-thread_local std::string g_err;
-
-int fail(int code, const std::string &msg) {
-    g_err = msg;
-    return code;
-}
-
-int hip_fail(hipError_t e, const char *what) {
-    std::string m = std::string(what) + ": " + hipGetErrorString(e);
-    return fail(e == hipErrorOutOfMemory ? LBM_ERR_NOMEM : LBM_ERR_HIP, m);
-}
-
-#define HIPCHK(expr)                                    \
-    do {                                                \
-        hipError_t e_ = (expr);                         \
-        if (e_ != hipSuccess) return hip_fail(e_, #expr); \
-    } while (0)
-
 size_t elem_size(int storage) { return storage == LBM_STORE_F64 ? 8 : storage == LBM_STORE_F32 ? 4 : 2; }
-
 }  // namespace
 
 struct lbm_ctx {
@@ -200,46 +183,15 @@ int step_impl(lbm_ctx *c, int variant, int nsteps) {
     }
 }
 
-// host node layout <-> device planes, interior nodes only
-int transfer_f64(lbm_ctx *c, double *host, double *devbuf, bool to_device) {
-    const Layout &L = c->prob.lay;
-    size_t plane_bytes = (size_t)L.plane * sizeof(double);
-    double *stage = nullptr;
-    HIPCHK(hipHostMalloc((void **)&stage, plane_bytes, hipHostMallocDefault));
-    int ny = L.ly + 2;
-    hipError_t e = hipSuccess;
-    for (int i = 0; i < 9 && e == hipSuccess; ++i) {
-        double *dev = devbuf + (int64_t)i * L.plane;
-        if (to_device) {
-            e = hipMemcpy(stage, dev, plane_bytes, hipMemcpyDeviceToHost);  // keep ghosts
-            if (e != hipSuccess) break;
-            for (int x = 1; x <= L.lx; ++x)
-                for (int y = 1; y <= L.ly; ++y)
-                    stage[L.node(x, y)] = host[((size_t)x * ny + y) * 9 + i];
-            e = hipMemcpy(dev, stage, plane_bytes, hipMemcpyHostToDevice);
-        } else {
-            e = hipMemcpy(stage, dev, plane_bytes, hipMemcpyDeviceToHost);
-            if (e != hipSuccess) break;
-            for (int x = 1; x <= L.lx; ++x)
-                for (int y = 1; y <= L.ly; ++y)
-                    host[((size_t)x * ny + y) * 9 + i] = stage[L.node(x, y)];
-        }
-    }
-    hipError_t e2 = hipHostFree(stage);
-    if (e != hipSuccess) return hip_fail(e, "population transfer");
-    if (e2 != hipSuccess) return hip_fail(e2, "hipHostFree");
-    return LBM_OK;
-}
-
 // host node layout (FP64) <-> device buffer k of any storage
 int transfer(lbm_ctx *c, double *host, int k, bool to_device) {
-    if (c->storage == LBM_STORE_F64) return transfer_f64(c, host, dbuf(c, k), to_device);
+    if (c->storage == LBM_STORE_F64) return transfer_f64(c->prob.lay, host, dbuf(c, k), to_device);
     const Layout &L = c->prob.lay;
     double *tmp = nullptr;
     HIPCHK(hipMalloc((void **)&tmp, 9 * (size_t)L.plane * sizeof(double)));
     hipError_t e = launch_widen(c->storage, L.plane, c->buf[k], tmp, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    int rc = e == hipSuccess ? transfer_f64(c, host, tmp, to_device) : hip_fail(e, "widen");
+    int rc = e == hipSuccess ? transfer_f64(c->prob.lay, host, tmp, to_device) : hip_fail(e, "widen");
     if (!rc && to_device) {
         e = launch_narrow(c->storage, L.plane, tmp, c->buf[k], c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -253,7 +205,7 @@ int transfer(lbm_ctx *c, double *host, int k, bool to_device) {
 
 extern "C" {
 
-const char *lbm_hip_last_error(void) { return g_err.c_str(); }
+const char *lbm_hip_last_error(void) { return lbm_g_err.c_str(); }
 
 size_t lbm_hip_footprint_of(const lbm_problem *p) {
     if (!p || p->lx < 1 || p->ly < 1) return 0;
@@ -316,7 +268,7 @@ int lbm_hip_create(const lbm_problem *p, lbm_ctx **out) {
     if (e != hipSuccess) return cleanup(e, "hipMalloc out");
     for (int s = 0; s < 4; ++s) {
         rc = upload_side(c, s, p, p->side[s].p0, p->side[s].p1);
-        if (rc) { std::string m = g_err; lbm_hip_destroy(c); return fail(rc, m); }
+        if (rc) { std::string m = lbm_g_err; lbm_hip_destroy(c); return fail(rc, m); }
         c->prob.side[s].kind = p->side[s].kind;
         c->prob.side[s].family = p->side[s].family;
         c->prob.side[s].p0 = c->d_p0[s];

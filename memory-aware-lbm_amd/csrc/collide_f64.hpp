@@ -42,6 +42,7 @@ __device__ __forceinline__ void collide_class(const Problem &p, int s, int x, in
 // zero-gradient outlet density from two post-stream densities
 // (updateZeroGradientBoundary, unsteady.c:211)
 __device__ __forceinline__ double zgb_rho(double rho1, double rho2) {
+    FP64_EXACT;
     return 4. / 3. * rho1 - 1. / 3. * rho2;
 }
 

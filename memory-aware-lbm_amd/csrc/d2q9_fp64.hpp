@@ -13,6 +13,10 @@
 
 namespace lbm {
 
+// Every FP64 operator below rounds each operation exactly once, whatever
+// -ffp-contract the including translation unit uses.
+#define FP64_EXACT _Pragma("clang fp contract(off)")
+
 // D2Q9 lattice (lb.c:48-55): 0 rest, 1 E, 2 N, 3 W, 4 S, 5 NE, 6 NW, 7 SW, 8 SE.
 __host__ __device__ constexpr int cx(int i) {
     return (i == 1 || i == 5 || i == 8) ? 1 : (i == 3 || i == 6 || i == 7) ? -1 : 0;
@@ -32,6 +36,7 @@ enum Side : int { UPPER = 0, LOWER = 1, LEFT = 2, RIGHT = 3 };
 // Moments (lb.c:134-141): density summed by lines of constant c_y; the
 // velocity by true IEEE division.
 __device__ __forceinline__ double rho_of(const double f[9]) {
+    FP64_EXACT;
     double north = f[2] + f[5] + f[6];
     double mid = f[0] + f[1] + f[3];
     double south = f[4] + f[7] + f[8];
@@ -40,6 +45,7 @@ __device__ __forceinline__ double rho_of(const double f[9]) {
 
 __device__ __forceinline__ void macros(const double f[9], double &rho, double &ux,
                                        double &uy) {
+    FP64_EXACT;
     double north = f[2] + f[5] + f[6];
     double mid = f[0] + f[1] + f[3];
     double south = f[4] + f[7] + f[8];
@@ -53,6 +59,7 @@ __device__ __forceinline__ void macros(const double f[9], double &rho, double &u
 // same way as cx*ux + cy*uy.
 template <int I>
 __device__ __forceinline__ double cdotu(double ux, double uy) {
+    FP64_EXACT;
     if constexpr (I == 0) return 0.;
     else if constexpr (I == 1) return ux;
     else if constexpr (I == 2) return uy;
@@ -66,6 +73,7 @@ __device__ __forceinline__ double cdotu(double ux, double uy) {
 
 template <int I>
 __device__ __forceinline__ double feq(double rho, double ux, double uy, double usqr) {
+    FP64_EXACT;
     double cu = cdotu<I>(ux, uy);
     return rho * wgt(I) * (1. + 3. * cu + 4.5 * cu * cu - 1.5 * usqr);
 }
@@ -73,19 +81,23 @@ __device__ __forceinline__ double feq(double rho, double ux, double uy, double u
 template <int... Is> struct seq {};
 template <typename F, int... Is>
 __device__ __forceinline__ void for_pops_impl(F &&fn, seq<Is...>) {
+    FP64_EXACT;
     (fn.template operator()<Is>(), ...);
 }
 template <typename F> __device__ __forceinline__ void for_pops(F &&fn) {
+    FP64_EXACT;
     for_pops_impl(fn, seq<0, 1, 2, 3, 4, 5, 6, 7, 8>{});
 }
 
 // BGK relaxation (lb.c:154-166): f_i = f_i*(1-omega), then += omega*feq_i.
 __device__ __forceinline__ void bgk(double f[9], double omega) {
+    FP64_EXACT;
     double rho, ux, uy;
     macros(f, rho, ux, uy);
     double usqr = ux * ux + uy * uy;
     double keep = 1 - omega;
     for_pops([&]<int I>() {
+        FP64_EXACT;
         f[I] *= keep;
         f[I] += omega * feq<I>(rho, ux, uy, usqr);
     });
@@ -93,6 +105,7 @@ __device__ __forceinline__ void bgk(double f[9], double omega) {
 
 // Full-way bounce-back (boundaries.c:48-59).
 __device__ __forceinline__ void bounce_back(double f[9]) {
+    FP64_EXACT;
     double t;
     t = f[1]; f[1] = f[3]; f[3] = t;
     t = f[2]; f[2] = f[4]; f[4] = t;
@@ -107,6 +120,7 @@ __device__ __forceinline__ void bounce_back(double f[9]) {
 // Known-population sum at a wall node (boundaries.c:66-126 bracket it as
 // tangential triple plus twice the outgoing triple).
 template <int S> __device__ __forceinline__ double wall_sum(const double f[9]) {
+    FP64_EXACT;
     if constexpr (S == UPPER) return f[0] + f[3] + f[1] + 2 * (f[6] + f[2] + f[5]);
     else if constexpr (S == LOWER) return f[0] + f[3] + f[1] + 2 * (f[8] + f[4] + f[7]);
     else if constexpr (S == RIGHT) return f[0] + f[2] + f[4] + 2 * (f[1] + f[5] + f[8]);
@@ -116,6 +130,7 @@ template <int S> __device__ __forceinline__ double wall_sum(const double f[9]) {
 // Density from a prescribed normal velocity (upperRho, lowerRho, rightRho,
 // leftRho: boundaries.c:66-118): reciprocal first, then the product.
 template <int S> __device__ __forceinline__ double wall_rho(const double f[9], double un) {
+    FP64_EXACT;
     double den = (S == UPPER || S == RIGHT) ? 1. + un : 1. - un;
     return 1. / den * wall_sum<S>(f);
 }
@@ -123,6 +138,7 @@ template <int S> __device__ __forceinline__ double wall_rho(const double f[9], d
 // Normal velocity from a prescribed density (upperU, lowerU, rightU, leftU:
 // boundaries.c:74-126).
 template <int S> __device__ __forceinline__ double wall_un(const double f[9], double rho) {
+    FP64_EXACT;
     double q = 1. / rho * wall_sum<S>(f);
     return (S == UPPER || S == RIGHT) ? -1. + q : 1. - q;
 }
@@ -130,6 +146,7 @@ template <int S> __device__ __forceinline__ double wall_un(const double f[9], do
 // Zou/He completion of the three unknown populations (boundaries.c:131-169).
 template <int S>
 __device__ __forceinline__ void zh_complete(double f[9], double ux, double uy, double rho) {
+    FP64_EXACT;
     if constexpr (S == UPPER) {
         f[7] = f[5] + 0.5 * (f[1] - f[3]) - rho * uy / 6 - rho * ux / 2.;
         f[8] = f[6] + 0.5 * (f[3] - f[1]) - rho * uy / 6 + rho * ux / 2.;
@@ -154,6 +171,7 @@ __device__ __forceinline__ void zh_complete(double f[9], double ux, double uy, d
 template <int S>
 __device__ __forceinline__ void neq_pi(const double n[9], double &pxx, double &pyy,
                                        double &pxy) {
+    FP64_EXACT;
     if constexpr (S == UPPER) {
         pxx = n[1] + n[3] + 2. * (n[5] + n[6]);
         pyy = 2. * (n[6] + n[2] + n[5]);
@@ -178,6 +196,7 @@ __device__ __forceinline__ void neq_pi(const double n[9], double &pxx, double &p
 // (cy^2-1/3) Pyy + 2 cx cy Pxy), the bracket summed left to right.
 template <int I>
 __device__ __forceinline__ double reg_term(double pxx, double pyy, double pxy) {
+    FP64_EXACT;
     constexpr double qx = (double)(cx(I) * cx(I)) - 1. / 3.;
     constexpr double qy = (double)(cy(I) * cy(I)) - 1. / 3.;
     constexpr double qxy = 2. * cx(I) * cy(I);
@@ -186,15 +205,20 @@ __device__ __forceinline__ double reg_term(double pxx, double pyy, double pxy) {
 
 template <int S>
 __device__ __forceinline__ void regularize(double f[9], double rho, double ux, double uy) {
+    FP64_EXACT;
     double usqr = ux * ux + uy * uy;
     double eq[9], neq[9];
     for_pops([&]<int I>() {
+        FP64_EXACT;
         eq[I] = feq<I>(rho, ux, uy, usqr);
         neq[I] = f[I] - eq[I];
     });
     double pxx, pyy, pxy;
     neq_pi<S>(neq, pxx, pyy, pxy);
-    for_pops([&]<int I>() { f[I] = eq[I] + reg_term<I>(pxx, pyy, pxy); });
+    for_pops([&]<int I>() {
+        FP64_EXACT;
+        f[I] = eq[I] + reg_term<I>(pxx, pyy, pxy);
+    });
 }
 
 // Wall operator families and kinds of a channel side.
@@ -207,6 +231,7 @@ enum BcKind : int { BC_VELOCITY = 0, BC_PRESSURE = 1 };
 template <int S>
 __device__ __forceinline__ void wall_node(double f[9], int kind, int family, double p0,
                                           double p1, double omega) {
+    FP64_EXACT;
     constexpr bool normal_y = (S == UPPER || S == LOWER);
     if (kind == BC_VELOCITY) {
         double rho = wall_rho<S>(f, normal_y ? p1 : p0);

@@ -1,0 +1,62 @@
+// host_util.hpp -- host-side helpers shared by the C-ABI translation units:
+// error reporting and the node-layout <-> device-plane transfer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/lbm_hip.h"
+#include "lbm_common.hpp"
+
+extern thread_local std::string lbm_g_err;
+
+
+inline int fail(int code, const std::string &msg) {
+    lbm_g_err = msg;
+    return code;
+}
+
+inline int hip_fail(hipError_t e, const char *what) {
+    std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    return fail(e == hipErrorOutOfMemory ? LBM_ERR_NOMEM : LBM_ERR_HIP, m);
+}
+
+#define HIPCHK(expr)                                    \
+    do {                                                \
+        hipError_t e_ = (expr);                         \
+        if (e_ != hipSuccess) return hip_fail(e_, #expr); \
+    } while (0)
+
+
+
+// host node layout <-> device planes, interior nodes only
+inline int transfer_f64(const lbm::Layout &L, double *host, double *devbuf, bool to_device) {
+    size_t plane_bytes = (size_t)L.plane * sizeof(double);
+    double *stage = nullptr;
+    HIPCHK(hipHostMalloc((void **)&stage, plane_bytes, hipHostMallocDefault));
+    int ny = L.ly + 2;
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < 9 && e == hipSuccess; ++i) {
+        double *dev = devbuf + (int64_t)i * L.plane;
+        if (to_device) {
+            e = hipMemcpy(stage, dev, plane_bytes, hipMemcpyDeviceToHost);  // keep ghosts
+            if (e != hipSuccess) break;
+            for (int x = 1; x <= L.lx; ++x)
+                for (int y = 1; y <= L.ly; ++y)
+                    stage[L.node(x, y)] = host[((size_t)x * ny + y) * 9 + i];
+            e = hipMemcpy(dev, stage, plane_bytes, hipMemcpyHostToDevice);
+        } else {
+            e = hipMemcpy(stage, dev, plane_bytes, hipMemcpyDeviceToHost);
+            if (e != hipSuccess) break;
+            for (int x = 1; x <= L.lx; ++x)
+                for (int y = 1; y <= L.ly; ++y)
+                    host[((size_t)x * ny + y) * 9 + i] = stage[L.node(x, y)];
+        }
+    }
+    hipError_t e2 = hipHostFree(stage);
+    if (e != hipSuccess) return hip_fail(e, "population transfer");
+    if (e2 != hipSuccess) return hip_fail(e2, "hipHostFree");
+    return LBM_OK;
+}
+

@@ -27,73 +27,17 @@
 #include "launch.hpp"
 #include "collide_f64.hpp"
 #include "fields.hpp"
+#include "node_step.hpp"
 
 namespace lbm {
-
-// Post-stream density of interior node (x, y): gathered from a collided
-// buffer, or read in place from a streamed one.
-template <bool PULL>
-__device__ __forceinline__ double density_at(const Layout &L, const double *__restrict__ s,
-                                             int x, int y) {
-    double g[9];
-    int64_t n = L.node(x, y);
-    for_pops([&]<int I>() {
-        g[I] = PULL ? s[I * L.plane + n - cx(I) * L.pitch - cy(I)] : s[I * L.plane + n];
-    });
-    return rho_of(g);
-}
-
-// Collision of one perimeter node, with the stale-slot bookkeeping and the
-// zero-gradient outlet density.  Kept out of line: perimeter nodes are a
-// thin minority and the bulk path should stay small.
-template <bool PULL>
-__device__ __forceinline__ void perimeter_node(const Problem &p, int s, int x, int y,
-                                            double f[9], const double *__restrict__ src,
-                                            double *__restrict__ dst) {
-    const Layout &L = p.lay;
-    int64_t n = L.node(x, y);
-    // Populations entering this node from outside the domain must be present
-    // again at the next step (the reference reads them at the four corners).
-    // Collided state: keep them in dst's ghost ring = this node's value of
-    // the step before (src(x)_i).  Streamed state: that value sits in dst(x)
-    // itself and is read before dst(x) is overwritten below.
-    for_pops([&]<int I>() {
-        if constexpr (I != 0) {
-            int gx = x - cx(I), gy = y - cy(I);
-            if (outside(L, gx, gy)) {
-                double prev = PULL ? src[I * L.plane + n] : dst[I * L.plane + n];
-                dst[I * L.plane + L.node(gx, gy)] = prev;
-            }
-        }
-    });
-    double right_p0 = 0.;
-    if (s == RIGHT && p.zgb) {
-        // updateZeroGradientBoundary (unsteady.c:205-214) on the post-stream
-        // state of this same step.
-        right_p0 = zgb_rho(density_at<PULL>(L, src, L.lx - 1, y),
-                           density_at<PULL>(L, src, L.lx - 2, y));
-    }
-    if (s == RIGHT) p.outlet_rho[y] = p.zgb ? right_p0 : p.side[RIGHT].p0[y];
-    collide_class(p, s, x, y, f, right_p0);
-}
 
 template <bool PULL>
 __global__ __launch_bounds__(256) void k_collide(Problem p, const double *__restrict__ src,
                                                  double *__restrict__ dst) {
-    const Layout &L = p.lay;
     int y = blockIdx.x * blockDim.x + threadIdx.x + 1;
     int x = blockIdx.y + 1;
-    if (y > L.ly) return;
-    int64_t n = L.node(x, y);
-    double f[9];
-    for_pops([&]<int I>() {
-        f[I] = PULL ? src[I * L.plane + n - cx(I) * L.pitch - cy(I)] : src[I * L.plane + n];
-    });
-    int s = side_of(p, x, y);
-    if (s >= 0) perimeter_node<PULL>(p, s, x, y, f, src, dst);
-    else if (p.r2 >= 0 && in_disc(p, x, y)) bounce_back(f);
-    else bgk(f, p.omega);
-    for_pops([&]<int I>() { __builtin_nontemporal_store(f[I], &dst[I * L.plane + n]); });
+    if (y > p.lay.ly) return;
+    node_step_f64<PULL>(p, x, y, src, dst);
 }
 
 __global__ __launch_bounds__(256) void k_stream(Layout L, const double *__restrict__ src,

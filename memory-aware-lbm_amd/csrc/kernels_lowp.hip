@@ -18,115 +18,18 @@
 #include "launch.hpp"
 #include "collide_f64.hpp"
 #include "fields.hpp"
+#include "lowp_ops.hpp"
+#include "node_step.hpp"
 
 namespace lbm {
-
-template <typename S> __device__ __forceinline__ float ld(const S *p) { return (float)*p; }
-template <typename S> __device__ __forceinline__ void st_nt(S v, S *p) {
-    __builtin_nontemporal_store(v, p);
-}
-
-template <int I> __device__ __forceinline__ float cdotu_f(float ux, float uy) {
-    if constexpr (I == 0) return 0.f;
-    else if constexpr (I == 1) return ux;
-    else if constexpr (I == 2) return uy;
-    else if constexpr (I == 3) return -ux;
-    else if constexpr (I == 4) return -uy;
-    else if constexpr (I == 5) return ux + uy;
-    else if constexpr (I == 6) return uy - ux;
-    else if constexpr (I == 7) return -ux - uy;
-    else return ux - uy;
-}
-
-// BGK on deviations, FP32.  rho = 1 + drho is never formed in FP32 (it would
-// drop the low bits of drho every step): u = j / (1 + drho) is evaluated as
-// j - j drho / (1 + drho) and rho X as X + drho X, so the rounding of 1 + drho
-// only touches terms that are already O(drho) small.
-__device__ __forceinline__ void bgk_dev(float g[9], float omega) {
-    float north = g[2] + g[5] + g[6];
-    float mid = g[0] + g[1] + g[3];
-    float south = g[4] + g[7] + g[8];
-    float drho = north + mid + south;
-    float jx = (g[1] + g[5] + g[8]) - (g[3] + g[6] + g[7]);
-    float jy = north - south;
-    float shrink = drho / (1.f + drho);
-    float ux = jx - jx * shrink, uy = jy - jy * shrink;
-    float usq = ux * ux + uy * uy;
-    // moving populations first; the rest population's equilibrium closes the
-    // mass balance exactly (sum geq = drho), so FP32 rounding of the
-    // equilibrium cannot bias the density step after step
-    float rest = drho;
-    for_pops([&]<int I>() {
-        if constexpr (I != 0) {
-            float cu = cdotu_f<I>(ux, uy);
-            float X = 3.f * cu + 4.5f * cu * cu - 1.5f * usq;
-            float geq = (float)wgt(I) * (drho + (X + drho * X));
-            rest -= geq;
-            g[I] += omega * (geq - g[I]);
-        }
-    });
-    g[0] += omega * (rest - g[0]);
-}
-
-__device__ __forceinline__ void bounce_dev(float g[9]) {
-    float t;
-    t = g[1]; g[1] = g[3]; g[3] = t;
-    t = g[2]; g[2] = g[4]; g[4] = t;
-    t = g[5]; g[5] = g[7]; g[7] = t;
-    t = g[6]; g[6] = g[8]; g[8] = t;
-}
-
-template <typename S, bool PULL>
-__device__ __forceinline__ double density_lp(const Layout &L, const S *__restrict__ s, int x,
-                                             int y) {
-    double f[9];
-    int64_t n = L.node(x, y);
-    for_pops([&]<int I>() {
-        float g = PULL ? ld(&s[I * L.plane + n - cx(I) * L.pitch - cy(I)]) : ld(&s[I * L.plane + n]);
-        f[I] = wgt(I) + (double)g;
-    });
-    return rho_of(f);
-}
-
-template <typename S, bool PULL>
-__device__ __forceinline__ void perimeter_lp(const Problem &p, int s, int x, int y, float g[9],
-                                          const S *__restrict__ src, S *__restrict__ dst) {
-    const Layout &L = p.lay;
-    int64_t n = L.node(x, y);
-    for_pops([&]<int I>() {
-        if constexpr (I != 0) {
-            int gx = x - cx(I), gy = y - cy(I);
-            if (outside(L, gx, gy)) dst[I * L.plane + L.node(gx, gy)] = PULL ? src[I * L.plane + n]
-                                                                             : dst[I * L.plane + n];
-        }
-    });
-    double f[9];
-    for_pops([&]<int I>() { f[I] = wgt(I) + (double)g[I]; });
-    double rp0 = 0.;
-    if (s == RIGHT && p.zgb)
-        rp0 = zgb_rho(density_lp<S, PULL>(L, src, L.lx - 1, y), density_lp<S, PULL>(L, src, L.lx - 2, y));
-    if (s == RIGHT) p.outlet_rho[y] = p.zgb ? rp0 : p.side[RIGHT].p0[y];
-    collide_class(p, s, x, y, f, rp0);
-    for_pops([&]<int I>() { g[I] = (float)(f[I] - wgt(I)); });
-}
 
 template <typename S, bool PULL>
 __global__ __launch_bounds__(256) void k_collide_lp(Problem p, const S *__restrict__ src,
                                                     S *__restrict__ dst) {
-    const Layout &L = p.lay;
     int y = blockIdx.x * blockDim.x + threadIdx.x + 1;
     int x = blockIdx.y + 1;
-    if (y > L.ly) return;
-    int64_t n = L.node(x, y);
-    float g[9];
-    for_pops([&]<int I>() {
-        g[I] = PULL ? ld(&src[I * L.plane + n - cx(I) * L.pitch - cy(I)]) : ld(&src[I * L.plane + n]);
-    });
-    int s = side_of(p, x, y);
-    if (s >= 0) perimeter_lp<S, PULL>(p, s, x, y, g, src, dst);
-    else if (p.r2 >= 0 && in_disc(p, x, y)) bounce_dev(g);
-    else bgk_dev(g, (float)p.omega);
-    for_pops([&]<int I>() { st_nt((S)g[I], &dst[I * L.plane + n]); });
+    if (y > p.lay.ly) return;
+    node_step_lp<S, PULL>(p, x, y, src, dst);
 }
 
 template <typename S>

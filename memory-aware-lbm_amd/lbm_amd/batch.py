@@ -1,0 +1,217 @@
+"""Batches of independent cylinder channels on one GPU (lbm_hip_batch_*), the
+seeded channel generator of the batching workload, and the launch-shape
+autotuner.
+
+Workload (the batching configuration): n channels from a seed, lx
+log-uniform in [128, 2048], ly = lx/4, disc centre x in [0.15, 0.3] lx and
+y in [0.4, 0.6] ly, radius in [0.05, 0.12] ly (integer truncation, so short
+channels get radius 1-3), Re uniform in [20, 200], u_max 0.02.
+
+The autotuner times every (tile-height cap, workgroups per CU) pair of a small
+grid on the actual batch with hipGraph replay and keeps the fastest; results
+are cached in JSON keyed by storage format and batch shape.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import hashlib
+import json
+import os
+import time
+from typing import Iterable, List, Optional
+
+import numpy as np
+
+from . import (BC_PRESSURE, BC_REGULARIZED, BC_VELOCITY, LEFT, LOWER, RIGHT, STORAGE, UPPER,
+               Channel, _P, _Problem, _SideBC, _check, _ptr, library)
+
+TUNE_DIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tuning")
+
+
+def channels_from_seed(n: int = 512, seed: int = 0, lx_lo: int = 128, lx_hi: int = 2048,
+                       re_lo: float = 20.0, re_hi: float = 200.0) -> List[Channel]:
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        lx = int(np.exp(rng.uniform(np.log(lx_lo), np.log(lx_hi))))
+        ly = max(lx // 4, 3)
+        ox = int(rng.uniform(0.15, 0.3) * lx)
+        oy = int(rng.uniform(0.4, 0.6) * ly)
+        r = max(int(rng.uniform(0.05, 0.12) * ly), 1)
+        re = float(rng.uniform(re_lo, re_hi))
+        out.append(Channel(lx, ly, ox, oy, r, 0.02, re))
+    return out
+
+
+def _problem(ch: Channel, storage: str, keep: list) -> _Problem:
+    u = ch.poiseuille()
+    inlet = u.copy()
+    inlet[[0, 1, ch.ly, ch.ly + 1]] = 0.0
+    zx, zy = np.zeros(ch.lx + 2), np.zeros(ch.ly + 2)
+    outlet = np.zeros(ch.ly + 2)
+    outlet[2:ch.ly] = 1.0
+    keep += [inlet, zx, zy, outlet]
+    p = _Problem()
+    p.lx, p.ly = ch.lx, ch.ly
+    p.obst_x, p.obst_y, p.obst_r = ch.obst_x, ch.obst_y, ch.obst_r
+    p.omega = ch.omega()
+    fam = ch.bc_family
+    p.side[UPPER] = _SideBC(BC_VELOCITY, fam, _ptr(zx), _ptr(zx))
+    p.side[LOWER] = _SideBC(BC_VELOCITY, fam, _ptr(zx), _ptr(zx))
+    p.side[LEFT] = _SideBC(BC_VELOCITY, fam, _ptr(inlet), _ptr(zy))
+    p.side[RIGHT] = _SideBC(BC_PRESSURE, fam, _ptr(outlet), _ptr(zy))
+    p.zgb = ch.zgb
+    p.storage = STORAGE[storage]
+    return p
+
+
+def _sig(lib):
+    vp = C.c_void_p
+    sigs = {
+        "lbm_hip_batch_create": (C.c_int, [C.POINTER(_Problem), C.c_int, C.POINTER(vp)]),
+        "lbm_hip_batch_destroy": (C.c_int, [vp]),
+        "lbm_hip_batch_init_equilibrium": (C.c_int, [vp, C.POINTER(_P)]),
+        "lbm_hip_batch_set_launch": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+        "lbm_hip_batch_step": (C.c_int, [vp, C.c_int]),
+        "lbm_hip_batch_sync": (C.c_int, [vp]),
+        "lbm_hip_batch_time": (C.c_int, [vp, C.c_int, _P]),
+        "lbm_hip_batch_download": (C.c_int, [vp, C.c_int, _P, _P, _P]),
+        "lbm_hip_batch_macros": (C.c_int, [vp, C.c_int, _P, _P, _P]),
+        "lbm_hip_batch_cells": (C.c_int64, [vp]),
+        "lbm_hip_batch_steps_done": (C.c_int64, [vp]),
+        "lbm_hip_batch_footprint": (C.c_size_t, [vp]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype, fn.argtypes = res, args
+    return lib
+
+
+class Batch:
+    """Many channels advanced together by one persistent launch per step."""
+
+    def __init__(self, channels: Iterable[Channel], storage: str = "f64"):
+        lib = _sig(library())
+        self.channels = list(channels)
+        self.storage = storage
+        self._keep: list = []
+        arr = (_Problem * len(self.channels))(*[_problem(c, storage, self._keep)
+                                                for c in self.channels])
+        h = C.c_void_p()
+        _check(lib.lbm_hip_batch_create(arr, len(self.channels), C.byref(h)), "batch_create")
+        self._h = h
+        self.launch = {"wgs_per_cu": 8, "ty_cap": 256, "graph_steps": 0}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            library().lbm_hip_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def cells(self) -> int:
+        return int(library().lbm_hip_batch_cells(self._h))
+
+    def footprint(self) -> int:
+        return int(library().lbm_hip_batch_footprint(self._h))
+
+    def init_equilibrium(self):
+        profiles = [np.ascontiguousarray(c.poiseuille()) for c in self.channels]
+        ptrs = (_P * len(profiles))(*[_ptr(p) for p in profiles])
+        _check(library().lbm_hip_batch_init_equilibrium(self._h, ptrs), "batch_init")
+        return self
+
+    def set_launch(self, wgs_per_cu: int = 0, ty_cap: int = 0, graph_steps: int = 0):
+        _check(library().lbm_hip_batch_set_launch(self._h, int(wgs_per_cu), int(ty_cap),
+                                                  int(graph_steps)), "batch_set_launch")
+        if wgs_per_cu:
+            self.launch["wgs_per_cu"] = wgs_per_cu
+        if ty_cap:
+            self.launch["ty_cap"] = ty_cap
+        self.launch["graph_steps"] = graph_steps
+
+    def step(self, n: int):
+        _check(library().lbm_hip_batch_step(self._h, int(n)), "batch_step")
+
+    def sync(self):
+        _check(library().lbm_hip_batch_sync(self._h), "batch_sync")
+
+    def time_steps(self, n: int) -> float:
+        ms = C.c_double()
+        _check(library().lbm_hip_batch_time(self._h, int(n), C.byref(ms)), "batch_time")
+        return ms.value
+
+    def steps_done(self) -> int:
+        return int(library().lbm_hip_batch_steps_done(self._h))
+
+    def download(self, i: int):
+        ch = self.channels[i]
+        lat = np.zeros((ch.lx + 2, ch.ly + 2, 9))
+        tmp = np.zeros_like(lat)
+        pr = np.zeros(ch.ly + 2)
+        _check(library().lbm_hip_batch_download(self._h, i, _ptr(lat), _ptr(tmp), _ptr(pr)),
+               "batch_download")
+        return lat, tmp, pr
+
+    def macros(self, i: int):
+        ch = self.channels[i]
+        n = ch.lx * ch.ly
+        r, u, v = np.zeros(n), np.zeros(n), np.zeros(n)
+        _check(library().lbm_hip_batch_macros(self._h, i, _ptr(r), _ptr(u), _ptr(v)),
+               "batch_macros")
+        s = (ch.lx, ch.ly)
+        return r.reshape(s), u.reshape(s), v.reshape(s)
+
+
+def batch_key(batch: Batch) -> str:
+    shape = ";".join(f"{c.lx}x{c.ly}" for c in batch.channels)
+    h = hashlib.sha1(shape.encode()).hexdigest()[:12]
+    return f"{batch.storage}-n{len(batch.channels)}-{h}"
+
+
+def autotune(batch: Batch, steps: int = 20, graph_steps: int = 10,
+             ty_caps=(32, 64, 128, 256), wgs=(2, 4, 8, 16),
+             cache_dir: Optional[str] = TUNE_DIR, use_cache: bool = True) -> dict:
+    """Pick the launch shape with the highest throughput on this batch.
+
+    Every candidate runs `steps` steps (after one warm-up replay) with
+    hipGraph replay; the state keeps advancing, which does not change the
+    per-step cost.  Returns {"ty_cap", "wgs_per_cu", "graph_steps", "glups",
+    "candidates": [...]}; cached by batch_key."""
+    key = batch_key(batch)
+    path = os.path.join(cache_dir, key + ".json") if cache_dir else None
+    if use_cache and path and os.path.exists(path):
+        best = json.load(open(path))
+        batch.set_launch(best["wgs_per_cu"], best["ty_cap"], best["graph_steps"])
+        best["cached"] = True
+        return best
+    cells = batch.cells
+    results = []
+    for ty in ty_caps:
+        for w in wgs:
+            batch.set_launch(w, ty, graph_steps)
+            batch.time_steps(graph_steps)  # capture + warm
+            ms = batch.time_steps(steps)
+            results.append({"ty_cap": ty, "wgs_per_cu": w,
+                            "glups": cells * steps / (ms * 1e-3) / 1e9})
+    best = max(results, key=lambda r: r["glups"])
+    out = {"ty_cap": best["ty_cap"], "wgs_per_cu": best["wgs_per_cu"],
+           "graph_steps": graph_steps, "glups": best["glups"], "key": key,
+           "tuned_at": time.strftime("%Y-%m-%d %H:%M:%S"), "candidates": results}
+    batch.set_launch(out["wgs_per_cu"], out["ty_cap"], graph_steps)
+    if path:
+        os.makedirs(cache_dir, exist_ok=True)
+        with open(path, "w") as fh:
+            json.dump(out, fh, indent=1)
+    return out
