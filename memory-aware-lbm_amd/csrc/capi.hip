@@ -36,6 +36,8 @@ struct lbm_ctx {
     hipStream_t stream;
     int tile_x, tile_y;
     size_t footprint;
+    int graph_steps;               // fused steps per captured graph (0: eager)
+    hipGraphExec_t graph[2];       // by starting buffer
 };
 
 namespace {
@@ -113,6 +115,33 @@ int to_streamed(lbm_ctx *c) {
     return LBM_OK;
 }
 
+void drop_graphs(lbm_ctx *c) {
+    for (int k = 0; k < 2; ++k)
+        if (c->graph[k]) {
+            (void)hipGraphExecDestroy(c->graph[k]);
+            c->graph[k] = nullptr;
+        }
+}
+
+// capture graph_steps fused pull steps starting from buffer `start`
+int capture_fused(lbm_ctx *c, int start) {
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+    hipError_t e = hipSuccess;
+    int cur = start;
+    for (int k = 0; k < c->graph_steps && e == hipSuccess; ++k) {
+        e = run_collide(c, true, cur, 1 - cur);
+        cur = 1 - cur;
+    }
+    hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+    if (e != hipSuccess) { if (g) (void)hipGraphDestroy(g); return hip_fail(e, "capture"); }
+    if (e2 != hipSuccess) return hip_fail(e2, "hipStreamEndCapture");
+    hipError_t e3 = hipGraphInstantiate(&c->graph[start], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e3 != hipSuccess) return hip_fail(e3, "hipGraphInstantiate");
+    return LBM_OK;
+}
+
 int step_impl(lbm_ctx *c, int variant, int nsteps) {
     if (nsteps < 0) return fail(LBM_ERR_ARG, "nsteps < 0");
     if (nsteps == 0) return LBM_OK;
@@ -135,6 +164,18 @@ int step_impl(lbm_ctx *c, int variant, int nsteps) {
             int rc = to_collided(c);
             if (rc) return rc;
             k = 1;
+        }
+        if (c->graph_steps > 0) {
+            while (nsteps - k >= c->graph_steps) {
+                if (!c->graph[c->cur]) {
+                    int rc = capture_fused(c, c->cur);
+                    if (rc) return rc;
+                }
+                HIPCHK(hipGraphLaunch(c->graph[c->cur], c->stream));
+                if (c->graph_steps & 1) c->cur = 1 - c->cur;
+                c->steps += c->graph_steps;
+                k += c->graph_steps;
+            }
         }
         for (; k < nsteps; ++k) {
             HIPCHK(run_collide(c, true, c->cur, 1 - c->cur));
@@ -286,6 +327,7 @@ int lbm_hip_create(const lbm_problem *p, lbm_ctx **out) {
 int lbm_hip_destroy(lbm_ctx *c) {
     if (!c) return LBM_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graphs(c);
     for (int k = 0; k < 2; ++k) if (c->buf[k]) (void)hipFree(c->buf[k]);
     for (int s = 0; s < 4; ++s) {
         if (c->d_p0[s]) (void)hipFree(c->d_p0[s]);
@@ -379,6 +421,30 @@ int lbm_hip_propagate(lbm_ctx *c) {
     if (!c) return fail(LBM_ERR_ARG, "null context");
     if (!c->collided) return fail(LBM_ERR_STATE, "propagate needs a collided state (collide first)");
     return to_streamed(c);
+}
+
+int lbm_hip_set_graph_steps(lbm_ctx *c, int graph_steps) {
+    if (!c || graph_steps < 0) return fail(LBM_ERR_ARG, "bad graph length");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (graph_steps != c->graph_steps) drop_graphs(c);
+    c->graph_steps = graph_steps;
+    return LBM_OK;
+}
+
+int lbm_hip_query(lbm_ctx *c) {
+    if (!c) return fail(LBM_ERR_ARG, "null context");
+    hipError_t e = hipStreamQuery(c->stream);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    return hip_fail(e, "hipStreamQuery");
+}
+
+int lbm_hip_device_memory(size_t *free_bytes, size_t *total_bytes) {
+    size_t f = 0, t = 0;
+    HIPCHK(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    return LBM_OK;
 }
 
 int lbm_hip_set_tile(lbm_ctx *c, int tile_x, int tile_y) {

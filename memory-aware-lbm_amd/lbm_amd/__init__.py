@@ -80,6 +80,9 @@ def library():
         "lbm_hip_footprint_of": (C.c_size_t, [C.POINTER(_Problem)]),
         "lbm_hip_apply_node_op": (C.c_int, [C.c_int, C.c_int, _P, _P, _P, C.c_double]),
         "lbm_hip_copy_bandwidth": (C.c_int, [C.c_size_t, C.c_int, _P]),
+        "lbm_hip_set_graph_steps": (C.c_int, [vp, C.c_int]),
+        "lbm_hip_query": (C.c_int, [vp]),
+        "lbm_hip_device_memory": (C.c_int, [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
         "lbm_hip_last_error": (C.c_char_p, []),
     }
     for name, (res, args) in sig.items():
@@ -219,6 +222,16 @@ class Lattice:
     def sync(self):
         _check(library().lbm_hip_sync(self._h), "sync")
 
+    def set_graph_steps(self, k: int):
+        _check(library().lbm_hip_set_graph_steps(self._h, int(k)), "set_graph_steps")
+
+    def done(self) -> bool:
+        """True when all enqueued work has finished (non-blocking)."""
+        rc = library().lbm_hip_query(self._h)
+        if rc < 0:
+            _check(rc, "query")
+        return rc == 1
+
     def steps_done(self) -> int:
         return int(library().lbm_hip_steps_done(self._h))
 
@@ -245,6 +258,20 @@ class Lattice:
 
     def footprint(self) -> int:
         return int(library().lbm_hip_footprint(self._h))
+
+
+def device_memory():
+    """(free, total) device bytes."""
+    f, t = C.c_size_t(), C.c_size_t()
+    _check(library().lbm_hip_device_memory(C.byref(f), C.byref(t)), "device_memory")
+    return int(f.value), int(t.value)
+
+
+def footprint_of(ch: "Channel", storage: str = "f64") -> int:
+    """Device bytes a context for this channel would hold (no allocation)."""
+    p = _Problem()
+    p.lx, p.ly, p.storage = ch.lx, ch.ly, STORAGE[storage]
+    return int(library().lbm_hip_footprint_of(C.byref(p)))
 
 
 def copy_bandwidth(nbytes: int, iters: int = 20) -> float:
