@@ -54,7 +54,7 @@ def test_admission_budget_and_slots():
 
 
 def test_late_arrival_overlaps_running_solve():
-    big = Request(0, 0.0, L.Channel(1024, 256, 200, 128, 20), 4000, "f64")
+    big = Request(0, 0.0, L.Channel(2048, 512, 400, 256, 40), 20000, "f64")
     small = Request(1, 0.05, L.Channel(64, 16, 12, 8, 2), 200, "f32")
     res = SolveServer(max_active=4, slice_ms=5.0, graph_steps=20).run([big, small])
     # the small solve is admitted while the big one runs and finishes first
