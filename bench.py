@@ -39,7 +39,7 @@ sys.path.insert(0, os.path.join(ROOT, "memory-aware-lbm_amd"))
 HBM_PEAK_GBS = 8000.0
 METRIC = "GLUPS & % of HBM roofline (FP64 vs FP32-store); serving p50/p99 latency"
 KERNEL_OF = {"origin": "k_collide<false>", "combine": "k_collide<true>",
-             "tight": "k_collide<true>", "tight_block": "k_block<64>", "panel": "k_panel"}
+             "tight": "k_collide<true>", "tight_block": "k_block<64>", "panel": "k_wpanel_in<double, 2>"}
 UPDATES_PER_LAUNCH = {"origin": 0.5, "combine": 1, "tight": 1, "tight_block": 2, "panel": 2}
 PROFILE = os.path.join(ROOT, "profiles", "fp64_kernels_16384x4096.md")
 

@@ -37,6 +37,8 @@ struct lbm_ctx {
     int tile_x, tile_y;
     size_t footprint;
     int graph_steps;               // fused steps per captured graph (0: eager)
+    hipStream_t side;              // second stream for the panel's edge waves
+    hipEvent_t fork, join;
     hipGraphExec_t graph[2];       // by starting buffer
 };
 
@@ -190,8 +192,8 @@ int step_impl(lbm_ctx *c, int variant, int nsteps) {
         // with one fused single step
         if (c->prob.lay.lx < 4)
             return fail(LBM_ERR_UNSUPPORTED, "two-step variants need lx >= 4");
-        if (c->storage != LBM_STORE_F64)
-            return fail(LBM_ERR_UNSUPPORTED, "two-step variants are built for FP64 storage only");
+        if (variant == LBM_TIGHT_BLOCK && c->storage != LBM_STORE_F64)
+            return fail(LBM_ERR_UNSUPPORTED, "tight_block is built for FP64 storage only");
         int k = 0;
         if (!c->collided) {
             int rc = to_collided(c);
@@ -200,9 +202,9 @@ int step_impl(lbm_ctx *c, int variant, int nsteps) {
         }
         for (; k + 2 <= nsteps; k += 2) {
             if (variant == LBM_PANEL) {
-                PanelGrid g = panel_grid(c->prob.lay, c->tile_y, c->tile_x);
-                HIPCHK(launch_panel_f64(c->prob, g, dbuf(c, c->cur), dbuf(c, 1 - c->cur),
-                                        c->stream));
+                WaveGrid g = wave_grid(c->prob.lay, c->tile_x);
+                HIPCHK(launch_wpanel(c->storage, c->prob, g, c->buf[c->cur], c->buf[1 - c->cur],
+                                     c->stream, c->side, c->fork, c->join));
             } else {
                 BlockGrid g = block_grid(c->prob.lay, c->tile_x,
                                          c->tile_y > 0 ? c->tile_y + 2 : 0);
@@ -281,6 +283,10 @@ int lbm_hip_create(const lbm_problem *p, lbm_ctx **out) {
     size_t bytes = 9 * (size_t)c->prob.lay.plane * elem_size(p->storage);
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete c; return hip_fail(e, "hipStreamCreate"); }
+    e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->join, hipEventDisableTiming);
+    if (e != hipSuccess) { lbm_hip_destroy(c); return hip_fail(e, "stream/event create"); }
     auto cleanup = [&](hipError_t err, const char *what) {
         lbm_hip_destroy(c);
         return hip_fail(err, what);
@@ -336,6 +342,10 @@ int lbm_hip_destroy(lbm_ctx *c) {
     if (c->d_outlet) (void)hipFree(c->d_outlet);
     if (c->d_partial) (void)hipFree(c->d_partial);
     if (c->d_out4) (void)hipFree(c->d_out4);
+    if (c->side) (void)hipStreamSynchronize(c->side);
+    if (c->fork) (void)hipEventDestroy(c->fork);
+    if (c->join) (void)hipEventDestroy(c->join);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return LBM_OK;

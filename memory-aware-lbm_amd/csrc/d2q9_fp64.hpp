@@ -167,33 +167,47 @@ __device__ __forceinline__ void zh_complete(double f[9], double ux, double uy, d
 }
 
 // Non-equilibrium momentum flux from the known populations (xNeqPi,
-// boundaries.c:266-302).
-template <int S>
-__device__ __forceinline__ void neq_pi(const double n[9], double &pxx, double &pyy,
-                                       double &pxy) {
+// boundaries.c:266-302).  n<I>() yields fNeq_I = f_I - feq_I (splitEqNeq,
+// boundaries.c:239-249); only the populations a side uses are formed.
+template <int S, typename N>
+__device__ __forceinline__ void neq_pi(N &&n, double &pxx, double &pyy, double &pxy) {
     FP64_EXACT;
     if constexpr (S == UPPER) {
-        pxx = n[1] + n[3] + 2. * (n[5] + n[6]);
-        pyy = 2. * (n[6] + n[2] + n[5]);
-        pxy = 2. * (n[5] - n[6]);
+        double n1 = n.template operator()<1>(), n2 = n.template operator()<2>();
+        double n3 = n.template operator()<3>(), n5 = n.template operator()<5>();
+        double n6 = n.template operator()<6>();
+        pxx = n1 + n3 + 2. * (n5 + n6);
+        pyy = 2. * (n6 + n2 + n5);
+        pxy = 2. * (n5 - n6);
     } else if constexpr (S == LOWER) {
-        pxx = n[1] + n[3] + 2. * (n[7] + n[8]);
-        pyy = 2. * (n[4] + n[7] + n[8]);
-        pxy = 2. * (n[7] - n[8]);
+        double n1 = n.template operator()<1>(), n3 = n.template operator()<3>();
+        double n4 = n.template operator()<4>(), n7 = n.template operator()<7>();
+        double n8 = n.template operator()<8>();
+        pxx = n1 + n3 + 2. * (n7 + n8);
+        pyy = 2. * (n4 + n7 + n8);
+        pxy = 2. * (n7 - n8);
     } else if constexpr (S == RIGHT) {
-        pxx = 2. * (n[1] + n[5] + n[8]);
-        pyy = n[2] + n[4] + 2. * (n[5] + n[8]);
-        pxy = 2. * (n[5] - n[8]);
+        double n1 = n.template operator()<1>(), n2 = n.template operator()<2>();
+        double n4 = n.template operator()<4>(), n5 = n.template operator()<5>();
+        double n8 = n.template operator()<8>();
+        pxx = 2. * (n1 + n5 + n8);
+        pyy = n2 + n4 + 2. * (n5 + n8);
+        pxy = 2. * (n5 - n8);
     } else {
-        pxx = 2. * (n[3] + n[6] + n[7]);
-        pyy = n[2] + n[4] + 2. * (n[6] + n[7]);
-        pxy = 2. * (n[7] - n[6]);
+        double n2 = n.template operator()<2>(), n3 = n.template operator()<3>();
+        double n4 = n.template operator()<4>(), n6 = n.template operator()<6>();
+        double n7 = n.template operator()<7>();
+        pxx = 2. * (n3 + n6 + n7);
+        pyy = n2 + n4 + 2. * (n6 + n7);
+        pxy = 2. * (n7 - n6);
     }
 }
 
 // Regularized reconstruction (splitEqNeq + xNeqPi + regularizedF,
 // boundaries.c:239-262): f_i = feq_i + 9/2 w_i ((cx^2-1/3) Pxx +
-// (cy^2-1/3) Pyy + 2 cx cy Pxy), the bracket summed left to right.
+// (cy^2-1/3) Pyy + 2 cx cy Pxy), the bracket summed left to right.  feq_i is
+// evaluated again for the reconstruction (the same operations, the same
+// bits) instead of being kept in nine registers.
 template <int I>
 __device__ __forceinline__ double reg_term(double pxx, double pyy, double pxy) {
     FP64_EXACT;
@@ -207,17 +221,15 @@ template <int S>
 __device__ __forceinline__ void regularize(double f[9], double rho, double ux, double uy) {
     FP64_EXACT;
     double usqr = ux * ux + uy * uy;
-    double eq[9], neq[9];
-    for_pops([&]<int I>() {
+    auto neq = [&]<int I>() {
         FP64_EXACT;
-        eq[I] = feq<I>(rho, ux, uy, usqr);
-        neq[I] = f[I] - eq[I];
-    });
+        return f[I] - feq<I>(rho, ux, uy, usqr);
+    };
     double pxx, pyy, pxy;
     neq_pi<S>(neq, pxx, pyy, pxy);
     for_pops([&]<int I>() {
         FP64_EXACT;
-        f[I] = eq[I] + reg_term<I>(pxx, pyy, pxy);
+        f[I] = feq<I>(rho, ux, uy, usqr) + reg_term<I>(pxx, pyy, pxy);
     });
 }
 

@@ -49,6 +49,7 @@ __device__ __forceinline__ bool solid_node(const Problem &p, int x, int y) {
 template <typename S, bool PULL>
 __global__ __launch_bounds__(256) void k_diag(Problem p, const S *__restrict__ src,
                                               double *__restrict__ partial) {
+    FP64_EXACT;
     const Layout &L = p.lay;
     int y = blockIdx.x * blockDim.x + threadIdx.x + 1;
     int x = blockIdx.y + 1;
@@ -58,6 +59,7 @@ __global__ __launch_bounds__(256) void k_diag(Problem p, const S *__restrict__ s
         load_post_stream<S, PULL>(L, src, x, y, f);
         bool solid = solid_node(p, x, y);
         if (!solid) {
+            FP64_EXACT;
             double r, u, v;
             macros(f, r, u, v);
             acc[2] = 0.5 * r * (u * u + v * v);

@@ -6,17 +6,12 @@
 // crosses the memory hierarchy once per two steps.  On the GPU the first
 // step's results live in LDS instead of cache:
 //
-//  k_panel  a workgroup owns a panel of rows y in [Y0, Y0+TY) and a chunk of
-//           columns x in [X0, X1); it marches x upward.  Iteration x does the
-//           first step (m) of column x into a 4-slot LDS ring and the second
-//           step (m+1) of column x-1 from the ring.  Four slots make one
-//           barrier per column enough (the slot written at x+1 is not read
-//           at x).  The next column's global loads are issued before the
-//           barrier and stay in flight across it.
 //  k_block  a workgroup owns a 2-D tile; the first step covers the tile plus
 //           a one-node ring in LDS, the second step the tile itself.
+//  (the panel sweep keeps its first step in wave registers instead:
+//   kernels_wave.hip)
 //
-// Both read only the collided buffer P (post-collision of step m-1, ghost
+// k_block reads only the collided buffer P (post-collision of step m-1, ghost
 // ring = unknown populations for step m) and write only Q (post-collision of
 // step m+1 and the ghost ring for step m+2), so workgroups never race.  The
 // first step's unknown populations come from P's ghost ring; the second
@@ -24,14 +19,14 @@
 // P(x) itself.  The outlet density of the second step is extrapolated from
 // the densities of columns lx-2 and lx-1 taken just before their second
 // collision, as the reference captures them (lb.c:968-977, 1025-1029);
-// every launch therefore gives the chunk/tile that holds column lx at least
-// three columns (chunks and tiles are aligned to the right end).
+// every launch therefore gives the tile that holds column lx at least three
+// columns (tiles are aligned to the right end).
 #include "launch.hpp"
 #include "collide_f64.hpp"
 
 namespace lbm {
 
-// LDS-only workgroup barrier: orders the ring's LDS traffic without waiting
+// LDS-only workgroup barrier: orders the tile's LDS traffic without waiting
 // for outstanding global loads (a plain __syncthreads() would drain them).
 __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
@@ -62,69 +57,6 @@ __device__ __forceinline__ void first_step(const Problem &p, const double *__res
         rp0 = zgb_rho(pulled_density(p.lay, P, p.lay.lx - 1, y),
                       pulled_density(p.lay, P, p.lay.lx - 2, y));
     collide_class(p, s, x, y, f, rp0);
-}
-
-constexpr int PANEL_THREADS = 256;
-
-__global__ __launch_bounds__(PANEL_THREADS) void k_panel(Problem p, PanelGrid g,
-                                                         const double *__restrict__ P,
-                                                         double *__restrict__ Q) {
-    __shared__ double ring[4][9][PANEL_THREADS];
-    const Layout &L = p.lay;
-    const int t = threadIdx.x;
-    const int py = blockIdx.x % g.npy;
-    const int ck = blockIdx.x / g.npy;
-    const int Y0 = 1 + py * g.ty;
-    const int y = Y0 - 1 + t;
-    const int X1 = L.lx + 1 - (g.ncx - 1 - ck) * g.cx;   // right-aligned chunks
-    const int X0 = X1 - g.cx > 1 ? X1 - g.cx : 1;
-    const bool row_in = y >= 1 && y <= L.ly;
-    const bool owner = t >= 1 && t <= g.ty && y <= L.ly;
-
-    double f[9], fn[9];
-    double rho1 = 0., rho2 = 0.;
-    if (row_in && X0 - 1 >= 1) pull_node(L, P, X0 - 1, y, f);
-
-    for (int x = X0 - 1; x <= X1; ++x) {
-        // ---- step m, column x -> ring
-        if (row_in && x >= 1 && x <= L.lx) {
-            first_step(p, P, x, y, f);
-            for_pops([&]<int I>() { ring[x & 3][I][t] = f[I]; });
-        }
-        if (row_in && x + 1 <= X1 && x + 1 <= L.lx) pull_node(L, P, x + 1, y, fn);
-        lds_barrier();
-        // ---- step m+1, column x-1 <- ring
-        const int xo = x - 1;
-        if (owner && xo >= X0) {
-            double h[9];
-            int64_t n = L.node(xo, y);
-            for_pops([&]<int I>() {
-                int sx = xo - cx(I), sy = y - cy(I);
-                h[I] = outside(L, sx, sy) ? P[I * L.plane + n] : ring[sx & 3][I][t - cy(I)];
-            });
-            // densities of columns lx-2, lx-1 before their second collision
-            if (xo == L.lx - 2) rho2 = rho_of(h);
-            if (xo == L.lx - 1) rho1 = rho_of(h);
-            int s = side_of(p, xo, y);
-            double rp0 = 0.;
-            if (s >= 0) {
-                // unknown populations of step m+2: this node's step-m values
-                for_pops([&]<int I>() {
-                    if constexpr (I != 0) {
-                        int gx = xo - cx(I), gy = y - cy(I);
-                        if (outside(L, gx, gy)) Q[I * L.plane + L.node(gx, gy)] = ring[xo & 3][I][t];
-                    }
-                });
-                if (s == RIGHT) {
-                    rp0 = zgb_rho(rho1, rho2);
-                    p.outlet_rho[y] = p.zgb ? rp0 : p.side[RIGHT].p0[y];
-                }
-            }
-            collide_class(p, s, xo, y, h, rp0);
-            for_pops([&]<int I>() { __builtin_nontemporal_store(h[I], &Q[I * L.plane + n]); });
-        }
-        for_pops([&]<int I>() { f[I] = fn[I]; });
-    }
 }
 
 // ---- 2-D tiles --------------------------------------------------------------
@@ -194,28 +126,6 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_block(Problem p, BlockGrid g,
 }
 
 // ---- launchers ----------------------------------------------------------------
-
-PanelGrid panel_grid(const Layout &L, int ty, int cx) {
-    PanelGrid g;
-    int max_ty = PANEL_THREADS - 2;
-    if (ty <= 0 || ty > max_ty) {
-        int npy = (L.ly + max_ty - 1) / max_ty;
-        ty = (L.ly + npy - 1) / npy;
-    }
-    if (cx < 3) cx = 128;
-    g.ty = ty;
-    g.cx = cx;
-    g.npy = (L.ly + ty - 1) / ty;
-    g.ncx = (L.lx + cx - 1) / cx;
-    return g;
-}
-
-hipError_t launch_panel_f64(const Problem &p, const PanelGrid &g, const double *P, double *Q,
-                            hipStream_t st) {
-    if (g.ty < 1 || g.ty > PANEL_THREADS - 2 || g.cx < 3) return hipErrorInvalidValue;
-    k_panel<<<g.npy * g.ncx, PANEL_THREADS, 0, st>>>(p, g, P, Q);
-    return hipGetLastError();
-}
 
 BlockGrid block_grid(const Layout &L, int tx, int typ) {
     BlockGrid g;

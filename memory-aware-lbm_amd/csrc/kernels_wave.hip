@@ -1,0 +1,351 @@
+// kernels_wave.hip -- two time steps per launch with the first step's results
+// held in registers of one wave64 (the panel sweep, collide_tight_panel_iy,
+// lb.c:1402-1538, mapped onto CDNA wavefronts).
+//
+// A wave owns a band of 62 rows and a chunk of columns, and marches x.  Its
+// 64 lanes cover the band plus one halo row on each side (lane l <-> row
+// Y0-1+l).  Iteration x computes the first step (m) of column x for all 64
+// rows into registers, then the second step (m+1) of column x-1 for the 62
+// owned rows.  The second step needs the first step's populations of
+// columns x-2, x-1 and x at rows y-1, y, y+1: a lane keeps its own rows'
+// values of the last two columns in registers and takes its neighbours'
+// with one lane shift up and one down (6 values per column).  Nothing goes
+// through LDS and no barrier is needed; waves are independent.
+//
+// Per column and lane: columns x-1 (all 9 populations: they are the second
+// step's cx = 0 sources next iteration and its unknown populations at the
+// walls) and x-2 (populations 1, 5, 8: the cx = +1 sources) stay resident.
+//
+// The second step's outlet density is extrapolated from the post-stream
+// densities of columns lx-2 and lx-1 gathered by the same lane just before
+// their second collision (lb.c:968-977); chunks are right-aligned so the
+// chunk holding lx also holds lx-2 and lx-1.  Unknown populations and the
+// ghost ring follow the collided-state convention of kernels_f64.hip: the
+// first step pulls them from P's ghost ring; the second step's are P(x)
+// itself; the output Q's ghost ring gets the first step's values.
+//
+// Storage S = double (bit-exact FP64) or float / _Float16 deviations (FP32
+// collisions, FP64 walls).  For reduced storage the first step's populations
+// are rounded to S exactly as a store would round them, so two steps here
+// equal two single steps bit for bit.
+#include "launch.hpp"
+#include "fields.hpp"
+#include "node_step.hpp"
+
+namespace lbm {
+
+template <typename S> struct Calc { using T = float; };
+template <> struct Calc<double> { using T = double; };
+
+template <typename S> __device__ __forceinline__ typename Calc<S>::T as_calc(S v) {
+    return (typename Calc<S>::T)v;
+}
+
+// round a computed value to what storage would hold, back in compute type
+template <typename S> __device__ __forceinline__ typename Calc<S>::T stored(typename Calc<S>::T v) {
+    if constexpr (sizeof(S) == 2) return (float)narrow<S>(v);
+    else return v;
+}
+
+__device__ __forceinline__ double lane_up(double v) { return __shfl_up(v, 1, 64); }
+__device__ __forceinline__ double lane_down(double v) { return __shfl_down(v, 1, 64); }
+__device__ __forceinline__ float lane_up(float v) { return __shfl_up(v, 1, 64); }
+__device__ __forceinline__ float lane_down(float v) { return __shfl_down(v, 1, 64); }
+
+// density of compute-type populations, summed in FP64 the way the
+// single-step kernels sum what they load
+template <typename S> __device__ __forceinline__ double rho_of_any(const typename Calc<S>::T h[9]) {
+    if constexpr (sizeof(S) == 8) {
+        return rho_of(h);
+    } else {
+        double f[9];
+        for_pops([&]<int I>() { f[I] = wgt(I) + (double)h[I]; });
+        return rho_of(f);
+    }
+}
+
+// post-stream density of interior node (x, y) gathered from P
+template <typename S>
+__device__ __forceinline__ double pulled_rho(const Layout &L, const S *__restrict__ P, int x, int y) {
+    double f[9];
+    int64_t n = L.node(x, y);
+    for_pops([&]<int I>() {
+        f[I] = pop_f64<S, I>(&P[I * L.plane + n - cx(I) * L.pitch - cy(I)]);
+    });
+    return rho_of(f);
+}
+
+// One node's collision on compute-type populations h, side s (-1: interior).
+template <typename S>
+__device__ __forceinline__ void collide_any(const Problem &p, int s, int x, int y,
+                                            typename Calc<S>::T h[9], double right_p0) {
+    if constexpr (sizeof(S) == 8) {
+        collide_class(p, s, x, y, h, right_p0);
+    } else {
+        if (s < 0) {
+            if (p.r2 >= 0 && in_disc(p, x, y)) bounce_dev(h);
+            else bgk_dev(h, (float)p.omega);
+            return;
+        }
+        double f[9];
+        for_pops([&]<int I>() { f[I] = wgt(I) + (double)h[I]; });
+        collide_class(p, s, x, y, f, right_p0);
+        for_pops([&]<int I>() { h[I] = (float)(f[I] - wgt(I)); });
+    }
+}
+
+// ---- perimeter waves ---------------------------------------------------------
+// Waves whose rows or columns (halo included) touch the channel perimeter
+// carry the wall operators, the outlet extrapolation and the unknown-population
+// bookkeeping.  One wave per workgroup; task e enumerates the frame of
+// (band, chunk) pairs around the interior rectangle.
+__device__ __forceinline__ void edge_task(const WaveGrid &g, int e, int &band, int &ck) {
+    const int nfull = g.b_lo + (g.nbands - g.b_hi);     // bands with every chunk on the edge
+    if (e < nfull * g.ncx) {
+        int i = e / g.ncx;
+        band = i < g.b_lo ? i : g.b_hi + (i - g.b_lo);
+        ck = e % g.ncx;
+    } else {
+        e -= nfull * g.ncx;
+        int ns = g.c_lo + (g.ncx - g.c_hi);              // edge chunks of an interior band
+        band = g.b_lo + e / ns;
+        int j = e % ns;
+        ck = j < g.c_lo ? j : g.c_hi + (j - g.c_lo);
+    }
+}
+
+template <typename S>
+__global__ __launch_bounds__(64) void k_wpanel_edge(Problem p, WaveGrid g,
+                                                    const S *__restrict__ P, S *__restrict__ Q) {
+    using T = typename Calc<S>::T;
+    const Layout &L = p.lay;
+    const int lane = threadIdx.x;
+    int band, ck;
+    edge_task(g, blockIdx.x, band, ck);
+    const WaveSpan w = wave_span(L, g, band, ck);
+    const int Y0 = w.Y0, X0 = w.X0, X1 = w.X1;
+    const int y = Y0 - 1 + lane;
+    const bool row_in = y >= 1 && y <= L.ly;
+    const bool owner = lane >= 1 && lane <= WAVE_ROWS && y <= L.ly;
+    const int s_row = (y == 1) ? LOWER : (y == L.ly) ? UPPER : -1;
+
+    T cur[9], c1[9], c2[3];      // columns x, x-1 and (1, 5, 8 of) x-2
+    S fn[9];                     // next column's pulled populations
+    for (int i = 0; i < 9; ++i) { cur[i] = 0; c1[i] = 0; fn[i] = 0; }
+    c2[0] = c2[1] = c2[2] = 0;
+    double rho1 = 0., rho2 = 0.;
+
+    auto pull = [&](int x, S f[9]) {
+        int64_t n = L.node(x, y);
+        for_pops([&]<int I>() { f[I] = P[I * L.plane + n - cx(I) * L.pitch - cy(I)]; });
+    };
+    if (row_in && X0 - 1 >= 1) pull(X0 - 1, fn);
+
+    for (int x = X0 - 1; x <= X1; ++x) {
+        // ---- step m, column x, all 64 rows -> registers
+        for_pops([&]<int I>() { cur[I] = as_calc<S>(fn[I]); });
+        if (row_in && x + 1 <= X1 && x + 1 <= L.lx) pull(x + 1, fn);
+        if (row_in && x >= 1 && x <= L.lx) {
+            int s = s_row >= 0 ? s_row : (x == 1 ? LEFT : (x == L.lx ? RIGHT : -1));
+            double rp0 = 0.;
+            if (s == RIGHT && p.zgb)
+                rp0 = zgb_rho(pulled_rho<S>(L, P, L.lx - 1, y), pulled_rho<S>(L, P, L.lx - 2, y));
+            collide_any<S>(p, s, x, y, cur, rp0);
+            for_pops([&]<int I>() { cur[I] = stored<S>(cur[I]); });
+        }
+        // ---- neighbour rows' first-step values (all lanes take part)
+        const T up2 = lane_up(c1[2]), up5 = lane_up(c2[1]), up6 = lane_up(cur[6]);
+        const T dn4 = lane_down(c1[4]), dn7 = lane_down(cur[7]), dn8 = lane_down(c2[2]);
+        // ---- step m+1, column x-1, owned rows
+        const int xo = x - 1;
+        if (owner && xo >= X0) {
+            T h[9] = {c1[0], c2[0], up2, cur[3], dn4, up5, up6, dn7, dn8};
+            int64_t n = L.node(xo, y);
+            int s = s_row >= 0 ? s_row : (xo == 1 ? LEFT : (xo == L.lx ? RIGHT : -1));
+            if (s >= 0) {
+                // populations entering from outside: this node's values of
+                // step m-1 (P itself); Q's ghost ring gets step m's
+                for_pops([&]<int I>() {
+                    if constexpr (I != 0) {
+                        int gx = xo - cx(I), gy = y - cy(I);
+                        if (outside(L, gx, gy)) {
+                            h[I] = as_calc<S>(P[I * L.plane + n]);
+                            Q[I * L.plane + L.node(gx, gy)] = (S)c1[I];
+                        }
+                    }
+                });
+            }
+            if (xo == L.lx - 2) rho2 = rho_of_any<S>(h);
+            if (xo == L.lx - 1) rho1 = rho_of_any<S>(h);
+            double rp0 = 0.;
+            if (s == RIGHT) {
+                rp0 = zgb_rho(rho1, rho2);
+                p.outlet_rho[y] = p.zgb ? rp0 : p.side[RIGHT].p0[y];
+            }
+            collide_any<S>(p, s, xo, y, h, rp0);
+            for_pops([&]<int I>() {
+                if constexpr (sizeof(S) == 8) __builtin_nontemporal_store(h[I], &Q[I * L.plane + n]);
+                else __builtin_nontemporal_store(narrow<S>(h[I]), &Q[I * L.plane + n]);
+            });
+        }
+        c2[0] = c1[1];
+        c2[1] = c1[5];
+        c2[2] = c1[8];
+        for_pops([&]<int I>() { c1[I] = cur[I]; });
+    }
+}
+
+// ---- interior waves -------------------------------------------------------------
+// Every lane is a channel row away from the walls and every column of the
+// march is an interior column, so the march is written without branches
+// around memory operations: each iteration issues the same loads and stores.
+// Lanes that must not store (the two halo rows, and every lane during the
+// two pipeline-filling iterations) get a buffer offset past the end of the
+// buffer resource, and the hardware drops their stores.  With identical
+// memory traffic on every path the compiler's wait for a column's loads
+// counts only the operations issued after them, so the stores are never
+// waited for and the loads of DEPTH columns stay in flight behind the
+// arithmetic.
+constexpr uint32_t DROP = 0x80000000u;        // offset that no resource here reaches
+constexpr int RSRC_WORD3 = 0x00020000;        // raw buffer descriptor word 3 (CDNA)
+constexpr int CPOL_NT = 2;                    // non-temporal store
+
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+template <typename S>
+__device__ __forceinline__ void buffer_store(S v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+    if constexpr (sizeof(S) == 8)
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, off, 0, CPOL_NT);
+    else if constexpr (sizeof(S) == 4)
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, off, 0, CPOL_NT);
+    else
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(uint16_t, v), r, off, 0, CPOL_NT);
+}
+
+template <typename S>
+__device__ __forceinline__ S to_storage(typename Calc<S>::T v) {
+    if constexpr (sizeof(S) == 8) return v;
+    else return narrow<S>(v);
+}
+
+// DEPTH = columns whose loads are in flight; (cx + 2) % DEPTH == 0.
+template <typename S, int DEPTH>
+__global__ __launch_bounds__(256) void k_wpanel_in(Problem p, WaveGrid g, const S *__restrict__ P,
+                                                   S *__restrict__ Q) {
+    using T = typename Calc<S>::T;
+    const Layout &L = p.lay;
+    const int lane = threadIdx.x & 63;
+    const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (t >= g.nin) return;                              // whole waves only
+    const int nbi = g.b_hi - g.b_lo;
+    const WaveSpan w = wave_span(L, g, g.b_lo + t % nbi, g.c_lo + t / nbi);
+    const int y = w.Y0 - 1 + lane;
+    const uint32_t colb = (uint32_t)L.pitch * (uint32_t)sizeof(S);
+    const uint32_t lane_off =
+        (lane >= 1 && lane <= WAVE_ROWS) ? (uint32_t)(y + L.yoff) * (uint32_t)sizeof(S) : DROP;
+    // per population: columns [X0, X1) of Q
+    __amdgpu_buffer_rsrc_t rs[9];
+    for_pops([&]<int I>() {
+        rs[I] = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)(Q + I * L.plane + (int64_t)w.X0 * L.pitch), 0,
+            (int)((uint32_t)(w.X1 - w.X0) * colb), RSRC_WORD3);
+    });
+
+    S f[DEPTH][9];
+    T cur[9], c1[9], c2[3];
+    for (int d = 0; d < DEPTH; ++d)
+        for (int i = 0; i < 9; ++i) f[d][i] = 0;
+    for (int i = 0; i < 9; ++i) { cur[i] = 0; c1[i] = 0; }
+    c2[0] = c2[1] = c2[2] = 0;
+
+    auto pull = [&](int x, S (&dst)[9]) {
+        const int64_t n = L.node(x, y);
+        for_pops([&]<int I>() { dst[I] = P[I * L.plane + n - cx(I) * L.pitch - cy(I)]; });
+    };
+    // iteration x: step m of column x, step m+1 of column x-1 (stored when
+    // x-1 >= X0); LOAD pulls column x + DEPTH into the buffer just consumed
+    auto iter = [&]<bool LOAD>(int x, S (&buf)[9]) {
+        for_pops([&]<int I>() { cur[I] = as_calc<S>(buf[I]); });
+        if constexpr (LOAD) pull(x + DEPTH, buf);
+        collide_any<S>(p, -1, x, y, cur, 0.);
+        for_pops([&]<int I>() { cur[I] = stored<S>(cur[I]); });
+        const T up2 = lane_up(c1[2]), up5 = lane_up(c2[1]), up6 = lane_up(cur[6]);
+        const T dn4 = lane_down(c1[4]), dn7 = lane_down(cur[7]), dn8 = lane_down(c2[2]);
+        T h[9] = {c1[0], c2[0], up2, cur[3], dn4, up5, up6, dn7, dn8};
+        collide_any<S>(p, -1, x - 1, y, h, 0.);
+        const uint32_t off = x - 1 >= w.X0 ? lane_off + (uint32_t)(x - 1 - w.X0) * colb : DROP;
+        for_pops([&]<int I>() { buffer_store<S>(to_storage<S>(h[I]), rs[I], off); });
+        c2[0] = c1[1];
+        c2[1] = c1[5];
+        c2[2] = c1[8];
+        for_pops([&]<int I>() { c1[I] = cur[I]; });
+    };
+
+    // iterations X0-1-DEPTH .. X1-DEPTH load; the first DEPTH of them run on
+    // zeros and only fill the pipeline (their stores are dropped, and their
+    // values leave the registers before column X0's second step reads them)
+    int x = w.X0 - 1 - DEPTH;
+    const int nload = (w.X1 - w.X0 + 2) / DEPTH;
+    for (int k = 0; k < nload; ++k, x += DEPTH) {
+        iter.template operator()<true>(x, f[0]);
+        if constexpr (DEPTH == 2) iter.template operator()<true>(x + 1, f[1]);
+    }
+    iter.template operator()<false>(x, f[0]);
+    if constexpr (DEPTH == 2) iter.template operator()<false>(x + 1, f[1]);
+}
+
+// Chunk width 16 unless given: short chunks put more waves in flight, which
+// is what a latency-bound march needs, at (cx + 2) / cx first-step work
+// (measured on 16384 x 4096: 16 beats 32, 64 and 128 for every storage).
+WaveGrid wave_grid(const Layout &L, int cx) {
+    WaveGrid g;
+    g.cx = cx >= 3 ? cx : 16;
+    g.nbands = (L.ly + WAVE_ROWS - 1) / WAVE_ROWS;
+    g.ncx = (L.lx + g.cx - 1) / g.cx;
+    g.ntasks = g.nbands * g.ncx;
+    // a wave is interior when its rows (halo included) stay off rows 1 and ly
+    // and its columns off columns 1 and lx: a band condition and a chunk one
+    int b_lo = g.nbands, b_hi = 0, c_lo = g.ncx, c_hi = 0;
+    for (int b = 0; b < g.nbands; ++b) {
+        const WaveSpan w = wave_span(L, g, b, 0);
+        if (w.Y0 - 1 > 1 && w.Y0 + WAVE_ROWS < L.ly) { b_lo = b < b_lo ? b : b_lo; b_hi = b + 1; }
+    }
+    for (int c = 0; c < g.ncx; ++c) {
+        const WaveSpan w = wave_span(L, g, 0, c);
+        if (w.X0 - 1 > 1 && w.X1 < L.lx) { c_lo = c < c_lo ? c : c_lo; c_hi = c + 1; }
+    }
+    if (b_lo >= b_hi || c_lo >= c_hi) b_lo = b_hi = c_lo = c_hi = 0;
+    g.b_lo = b_lo; g.b_hi = b_hi; g.c_lo = c_lo; g.c_hi = c_hi;
+    g.nin = (b_hi - b_lo) * (c_hi - c_lo);
+    g.nedge = g.ntasks - g.nin;
+    return g;
+}
+
+// Perimeter waves on `side`, interior waves on `st`, joined back into `st`:
+// the two kernels write disjoint nodes and run concurrently.
+hipError_t launch_wpanel(int storage, const Problem &p, const WaveGrid &g, const void *P, void *Q,
+                         hipStream_t st, hipStream_t side, hipEvent_t fork, hipEvent_t join) {
+    if (g.cx < 3) return hipErrorInvalidValue;
+    const bool deep = (g.cx + 2) % 2 == 0;
+    const int blocks = (g.nin + 3) / 4;
+    hipError_t e = hipEventRecord(fork, st);
+    if (e == hipSuccess) e = hipStreamWaitEvent(side, fork, 0);
+    if (e != hipSuccess) return e;
+    auto go = [&](auto tag) {
+        using S = decltype(tag);
+        if (g.nedge) k_wpanel_edge<S><<<g.nedge, 64, 0, side>>>(p, g, (const S *)P, (S *)Q);
+        if (g.nin) {
+            if (deep) k_wpanel_in<S, 2><<<blocks, 256, 0, st>>>(p, g, (const S *)P, (S *)Q);
+            else k_wpanel_in<S, 1><<<blocks, 256, 0, st>>>(p, g, (const S *)P, (S *)Q);
+        }
+    };
+    if (storage == 0) go(double{});
+    else if (storage == 1) go(float{});
+    else go(_Float16{});
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventRecord(join, side);
+    if (e == hipSuccess) e = hipStreamWaitEvent(st, join, 0);
+    return e;
+}
+
+}  // namespace lbm

@@ -39,23 +39,39 @@ hipError_t launch_widen(int storage, int64_t plane, const void *src, double *dst
 hipError_t launch_narrow(int storage, int64_t plane, const double *src, void *dst, hipStream_t st);
 
 // Two-step (temporally blocked) kernels: kernels_2step_f64.hip
-struct PanelGrid {
-    int ty;   // rows owned per panel (<= 254)
-    int cx;   // columns per chunk (>= 3)
-    int npy, ncx;
-};
 struct BlockGrid {
     int tx;   // tile columns (>= 3)
     int typ;  // tile rows + 2 (32, 64 or 128)
     int nbx, nby;
 };
-PanelGrid panel_grid(const Layout &L, int ty, int cx);
 BlockGrid block_grid(const Layout &L, int tx, int typ);
 size_t block_lds_bytes(const BlockGrid &g);
-hipError_t launch_panel_f64(const Problem &p, const PanelGrid &g, const double *P, double *Q,
-                            hipStream_t st);
 hipError_t launch_block_f64(const Problem &p, const BlockGrid &g, const double *P, double *Q,
                             hipStream_t st);
 hipError_t block_kernel_setup();
+
+// Two steps per launch, first step held in wave64 registers: kernels_wave.hip
+constexpr int WAVE_ROWS = 62;   // owned rows per wave (64 lanes incl. a halo row each side)
+struct WaveGrid {
+    int cx;                        // columns per chunk (>= 3)
+    int nbands, ncx, ntasks;
+    int b_lo, b_hi, c_lo, c_hi;    // (band, chunk) rectangle of the interior waves
+    int nin, nedge;                // interior waves, perimeter waves
+};
+
+// Rows [Y0-1, Y0+62] (lanes) and columns [X0, X1) owned by wave (band, ck);
+// chunks are aligned to the right end of the channel.
+struct WaveSpan { int Y0, X0, X1; };
+__host__ __device__ inline WaveSpan wave_span(const Layout &L, const WaveGrid &g, int band, int ck) {
+    WaveSpan w;
+    w.Y0 = 1 + band * WAVE_ROWS;
+    w.X1 = L.lx + 1 - (g.ncx - 1 - ck) * g.cx;
+    w.X0 = w.X1 - g.cx > 1 ? w.X1 - g.cx : 1;
+    return w;
+}
+
+WaveGrid wave_grid(const Layout &L, int cx);
+hipError_t launch_wpanel(int storage, const Problem &p, const WaveGrid &g, const void *P, void *Q,
+                         hipStream_t st, hipStream_t side, hipEvent_t fork, hipEvent_t join);
 
 }  // namespace lbm

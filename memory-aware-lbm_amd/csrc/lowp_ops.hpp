@@ -8,6 +8,16 @@
 
 namespace lbm {
 
+// Narrow an FP32 result to the storage type.  For FP16 the value is pinned in
+// a register first: otherwise the compiler may fold the last FP32 fma and the
+// narrowing into one v_fma_mixlo_f16 (a single rounding instead of FP32 then
+// FP16), in some kernels and not in others, and the same node would get
+// different bits depending on the code around it.
+template <typename S> __device__ __forceinline__ S narrow(float v) {
+    if constexpr (sizeof(S) == 2) __asm__ volatile("" : "+v"(v));
+    return (S)v;
+}
+
 template <int I> __device__ __forceinline__ float cdotu_f(float ux, float uy) {
     if constexpr (I == 0) return 0.f;
     else if constexpr (I == 1) return ux;

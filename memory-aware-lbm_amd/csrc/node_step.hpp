@@ -128,7 +128,7 @@ __device__ __forceinline__ void node_step_lp(const Problem &p, int x, int y,
     if (s >= 0) perimeter_lp<S, PULL>(p, s, x, y, g, src, dst);
     else if (p.r2 >= 0 && in_disc(p, x, y)) bounce_dev(g);
     else bgk_dev(g, (float)p.omega);
-    for_pops([&]<int I>() { st_nt((S)g[I], &dst[I * L.plane + n]); });
+    for_pops([&]<int I>() { st_nt(narrow<S>(g[I]), &dst[I * L.plane + n]); });
 }
 
 }  // namespace lbm

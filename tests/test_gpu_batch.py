@@ -29,9 +29,8 @@ def alone(ch, steps, storage):
 
 @pytest.mark.parametrize("storage", ["f64", "f32"])
 def test_batch_equals_single_contexts(storage):
-    """FP64: bit-identical.  FP32 storage: the batched and the single-lattice
-    kernels differ by one FP64 ulp in some wall-node values (an open item,
-    see DESIGN.md), so that form is held to 1e-12."""
+    """Bit-identical for every storage format: the batched and the
+    single-lattice kernels run the same node step."""
     chans = small_channels()
     steps = 23
     with Batch(chans, storage) as b:
@@ -42,10 +41,7 @@ def test_batch_equals_single_contexts(storage):
         want = alone(ch, steps, storage)
         for g, w, name in zip(got[i], want, ("lattice", "tmp", "outlet")):
             a, b_ = (g[2:ch.ly], w[2:ch.ly]) if name == "outlet" else (g[1:-1, 1:-1], w[1:-1, 1:-1])
-            if storage == "f64":
-                assert np.array_equal(a, b_), (i, name)
-            else:
-                assert np.abs(a - b_).max() <= 1e-12, (i, name, np.abs(a - b_).max())
+            assert np.array_equal(a, b_), (i, name, np.abs(a - b_).max())
 
 
 def test_batch_matches_oracle(orc):
