@@ -5,8 +5,8 @@ Re 100, u_max 0.02, regularized walls, Poiseuille inlet, zero-gradient
 pressure outlet -- with FP64 populations, bit-identical to the reference C
 code.  W warm-up steps, then K timed steps; one JSON line on stdout.
 
-`value` is the GLUPS of the headline sweep variant (default tight_block: two
-time steps per launch through LDS).  The other variants are timed on the same
+`value` is the GLUPS of the headline sweep variant (default panel: two time
+steps per launch, the first held in wave64 registers).  The other variants are timed on the same
 lattice and reported under "variants".  The populations are initialised on
 the device, so everything is resident in HBM when the timed region starts.
 
@@ -39,9 +39,14 @@ sys.path.insert(0, os.path.join(ROOT, "memory-aware-lbm_amd"))
 HBM_PEAK_GBS = 8000.0
 METRIC = "GLUPS & % of HBM roofline (FP64 vs FP32-store); serving p50/p99 latency"
 KERNEL_OF = {"origin": "k_collide<false>", "combine": "k_collide<true>",
-             "tight": "k_collide<true>", "tight_block": "k_block<64>", "panel": "k_wpanel_in<double, 2>"}
+             "tight": "k_collide<true>", "tight_block": "k_block<64>",
+             "panel": "k_wpanel_in<double, 2, 8>"}
+LOWP_KERNEL_OF = {("f32", "combine"): "k_collide_lp<float, true>",
+                  ("f16", "combine"): "k_collide_lp<_Float16, true>",
+                  ("f32", "panel"): "k_wpanel_in<float, 2, 8>",
+                  ("f16", "panel"): "k_wpanel_in<_Float16, 2, 8>"}
 UPDATES_PER_LAUNCH = {"origin": 0.5, "combine": 1, "tight": 1, "tight_block": 2, "panel": 2}
-PROFILE = os.path.join(ROOT, "profiles", "fp64_kernels_16384x4096.md")
+PROFILE = os.path.join(ROOT, "profiles", "kernels_16384x4096.md")
 
 
 def parse():
@@ -51,12 +56,13 @@ def parse():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--lx", type=int, default=16384)
     ap.add_argument("--ly", type=int, default=4096)
-    ap.add_argument("--variant", default="tight_block")
-    ap.add_argument("--variants", default="origin,combine,panel",
+    ap.add_argument("--variant", default="panel")
+    ap.add_argument("--variants", default="origin,combine,tight_block",
                     help="variants timed besides the headline one (comma list, or '')")
-    ap.add_argument("--other-steps", type=int, default=200)
+    ap.add_argument("--other-steps", type=int, default=1000)
     ap.add_argument("--storages", default="f32,f16",
-                    help="reduced-precision storage forms timed with the fused kernel")
+                    help="reduced-precision storage forms, timed with the fused and the "
+                         "panel sweeps")
     ap.add_argument("--batch", type=int, default=512,
                     help="channels of the batching workload (0 = skip)")
     ap.add_argument("--batch-seed", type=int, default=0)
@@ -85,7 +91,11 @@ def time_variant(lbm, ch, variant, steps, warmup, storage="f64"):
                 launch_ms=ms / launches)
 
 
-def measured_traffic(kernel):
+# kernels of one launch of a variant whose HBM traffic adds up to the launch's
+TRAFFIC_KERNELS = {"panel": ("k_wpanel_in<double, 2, 8>", "k_wpanel_edge<double>")}
+
+
+def measured_traffic(variant):
     """HBM bytes per launch from the committed rocprofv3 PMC summary."""
     try:
         text = open(PROFILE).read()
@@ -94,11 +104,14 @@ def measured_traffic(kernel):
     m = re.search(r"traffic_bytes_per_launch:\s*(.*)", text)
     if not m:
         return None
-    for item in m.group(1).split(","):
-        k, _, v = item.strip().partition("=")
-        if k == kernel:
-            return float(v)
-    return None
+    table = {}
+    for item in m.group(1).split(", "):
+        k, _, v = item.strip().rpartition("=")
+        table[k] = float(v)
+    names = TRAFFIC_KERNELS.get(variant, (KERNEL_OF[variant],))
+    if not all(n in table for n in names):
+        return None
+    return sum(table[n] for n in names)
 
 
 def batching(args):
@@ -219,19 +232,23 @@ def main():
                      "achieved_gbs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4)}
     storage = {}
     for sname in [v for v in args.storages.split(",") if v]:
-        r = time_variant(lbm, ch, "combine", args.other_steps, min(args.warmup, 20), sname)
         bpu = {"f32": 72.0, "f16": 36.0}[sname]
-        ach = bpu * cells / (r["launch_ms"] * 1e-3) / 1e9
-        storage[sname] = {"glups": round(r["glups"], 4), "ms_per_step": round(r["ms_per_step"], 5),
-                          "kernel": f"k_collide_lp<{sname}>", "bytes_per_update": bpu,
-                          "achieved_gbs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4)}
+        storage[sname] = {"bytes_per_update": bpu}
+        for v in ("combine", "panel"):
+            r = time_variant(lbm, ch, v, args.other_steps, min(args.warmup, 20), sname)
+            ach = bpu * cells * UPDATES_PER_LAUNCH[v] / (r["launch_ms"] * 1e-3) / 1e9
+            storage[sname][v] = {"glups": round(r["glups"], 4),
+                                 "ms_per_step": round(r["ms_per_step"], 5),
+                                 "kernel": LOWP_KERNEL_OF[(sname, v)],
+                                 "launch_ms": round(r["launch_ms"], 5),
+                                 "achieved_gbs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4)}
     batch = batching(args) if args.batch > 0 else None
     serve = serving(args) if args.serve > 0 else None
     copy_gbs = lbm.copy_bandwidth(2 << 30, 20)
     kernel = KERNEL_OF[args.variant]
     upl = UPDATES_PER_LAUNCH[args.variant]
     achieved = 144.0 * cells * upl / (head["launch_ms"] * 1e-3) / 1e9
-    traffic = measured_traffic(kernel) if (args.lx, args.ly) == (16384, 4096) else None
+    traffic = measured_traffic(args.variant) if (args.lx, args.ly) == (16384, 4096) else None
     out = {
         "metric": METRIC,
         "value": round(head["glups"], 4),

@@ -136,7 +136,10 @@ int lbm_hip_collide(lbm_ctx *ctx);
 int lbm_hip_propagate(lbm_ctx *ctx);
 
 /* Block-size knobs of the blocked variants (the reference's blk_size /
- * thread_block, unsteady.c:93,258): tile extent in x and y.  0 = default. */
+ * thread_block, unsteady.c:93,258).  TIGHT_BLOCK: tile extent in x and y.
+ * PANEL: tile_x = columns per wave's march (>= 3), tile_y = waves per
+ * workgroup marching in lockstep (4, 8 or 16; other values pick 8).
+ * 0 = default.  The results do not depend on either knob. */
 int lbm_hip_set_tile(lbm_ctx *ctx, int tile_x, int tile_y);
 
 /* Capture graph_steps consecutive fused steps (combine/tight variants) into

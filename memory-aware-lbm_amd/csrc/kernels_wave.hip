@@ -228,14 +228,27 @@ __device__ __forceinline__ S to_storage(typename Calc<S>::T v) {
     else return narrow<S>(v);
 }
 
-// DEPTH = columns whose loads are in flight; (cx + 2) % DEPTH == 0.
-template <typename S, int DEPTH>
-__global__ __launch_bounds__(256) void k_wpanel_in(Problem p, WaveGrid g, const S *__restrict__ P,
-                                                   S *__restrict__ Q) {
+// Workgroups are dealt to the 8 XCDs in turn; renumber them so that each XCD
+// takes a contiguous range of tasks: neighbouring bands, whose halo rows and
+// shared cache lines overlap, then meet in the same L2.
+__device__ __forceinline__ int xcd_block(int b, int nblk) {
+    const int xcd = b % 8, k = b / 8, per = nblk / 8, rem = nblk % 8;
+    return xcd < rem ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
+}
+
+// DEPTH = columns whose loads are in flight; (cx + 2) % DEPTH == 0.  WPG
+// waves per workgroup take consecutive bands; above 4 they march in lockstep
+// (one barrier per column), so the workgroup's loads of a column arrive at
+// the memory together as one contiguous run per population.
+template <typename S, int DEPTH, int WPG>
+__global__ __launch_bounds__(64 * WPG) void k_wpanel_in(Problem p, WaveGrid g,
+                                                        const S *__restrict__ P,
+                                                        S *__restrict__ Q) {
     using T = typename Calc<S>::T;
     const Layout &L = p.lay;
     const int lane = threadIdx.x & 63;
-    const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+    const int t = __builtin_amdgcn_readfirstlane(
+        (WPG > 4 ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x) * WPG + (threadIdx.x >> 6));
     if (t >= g.nin) return;                              // whole waves only
     const int nbi = g.b_hi - g.b_lo;
     const WaveSpan w = wave_span(L, g, g.b_lo + t % nbi, g.c_lo + t / nbi);
@@ -279,6 +292,7 @@ __global__ __launch_bounds__(256) void k_wpanel_in(Problem p, WaveGrid g, const 
         c2[1] = c1[5];
         c2[2] = c1[8];
         for_pops([&]<int I>() { c1[I] = cur[I]; });
+        if constexpr (WPG > 4) __builtin_amdgcn_s_barrier();
     };
 
     // iterations X0-1-DEPTH .. X1-DEPTH load; the first DEPTH of them run on
@@ -294,12 +308,15 @@ __global__ __launch_bounds__(256) void k_wpanel_in(Problem p, WaveGrid g, const 
     if constexpr (DEPTH == 2) iter.template operator()<false>(x + 1, f[1]);
 }
 
-// Chunk width 16 unless given: short chunks put more waves in flight, which
-// is what a latency-bound march needs, at (cx + 2) / cx first-step work
-// (measured on 16384 x 4096: 16 beats 32, 64 and 128 for every storage).
-WaveGrid wave_grid(const Layout &L, int cx) {
+// Chunk width 16 and 8 lockstep waves per workgroup unless given.  Short
+// chunks keep the last round of waves small and the march's pipeline-fill
+// share low enough: (cx + 2) / cx first-step work, (cx + 4) / cx iterations.
+// On 16384 x 4096 (FP64) 16 x 8 gives 59 GLUPS against 52.8 for 16 x 4
+// without lockstep and 55 for 128 x 16.
+WaveGrid wave_grid(const Layout &L, int cx, int wpg) {
     WaveGrid g;
     g.cx = cx >= 3 ? cx : 16;
+    g.wpg = (wpg == 4 || wpg == 16) ? wpg : 8;
     g.nbands = (L.ly + WAVE_ROWS - 1) / WAVE_ROWS;
     g.ncx = (L.lx + g.cx - 1) / g.cx;
     g.ntasks = g.nbands * g.ncx;
@@ -327,7 +344,7 @@ hipError_t launch_wpanel(int storage, const Problem &p, const WaveGrid &g, const
                          hipStream_t st, hipStream_t side, hipEvent_t fork, hipEvent_t join) {
     if (g.cx < 3) return hipErrorInvalidValue;
     const bool deep = (g.cx + 2) % 2 == 0;
-    const int blocks = (g.nin + 3) / 4;
+    const int blocks = (g.nin + g.wpg - 1) / g.wpg;
     hipError_t e = hipEventRecord(fork, st);
     if (e == hipSuccess) e = hipStreamWaitEvent(side, fork, 0);
     if (e != hipSuccess) return e;
@@ -335,8 +352,24 @@ hipError_t launch_wpanel(int storage, const Problem &p, const WaveGrid &g, const
         using S = decltype(tag);
         if (g.nedge) k_wpanel_edge<S><<<g.nedge, 64, 0, side>>>(p, g, (const S *)P, (S *)Q);
         if (g.nin) {
-            if (deep) k_wpanel_in<S, 2><<<blocks, 256, 0, st>>>(p, g, (const S *)P, (S *)Q);
-            else k_wpanel_in<S, 1><<<blocks, 256, 0, st>>>(p, g, (const S *)P, (S *)Q);
+            auto in = [&](auto depth, auto wpg) {
+                constexpr int D = decltype(depth)::value, W = decltype(wpg)::value;
+                k_wpanel_in<S, D, W><<<blocks, 64 * W, 0, st>>>(p, g, (const S *)P, (S *)Q);
+            };
+            using I1 = std::integral_constant<int, 1>;
+            using I2 = std::integral_constant<int, 2>;
+            using W4 = std::integral_constant<int, 4>;
+            using W8 = std::integral_constant<int, 8>;
+            using W16 = std::integral_constant<int, 16>;
+            if (deep) {
+                if (g.wpg == 8) in(I2{}, W8{});
+                else if (g.wpg == 16) in(I2{}, W16{});
+                else in(I2{}, W4{});
+            } else {
+                if (g.wpg == 8) in(I1{}, W8{});
+                else if (g.wpg == 16) in(I1{}, W16{});
+                else in(I1{}, W4{});
+            }
         }
     };
     if (storage == 0) go(double{});

@@ -140,6 +140,23 @@ def test_diagnostics_match_oracle(orc):
         assert abs(got[k] - want[k]) <= 1e-12 * max(scale, 1e-3), (k, got[k], want[k])
 
 
+def test_panel_interior_waves(orc):
+    """A lattice with several bands of interior waves: the branch-free
+    interior march (one and two columns of loads in flight, 4, 8 and 16
+    waves per workgroup) against the oracle, bit for bit."""
+    ch = OChannel.geometry(300, 260, "B")
+    _, lref, tref, pref = orc.run(ch, 13)
+    for cxk, wpg in ((16, 0), (17, 0), (32, 8), (33, 16), (20, 16), (3, 8), (5, 4)):
+        with L.Lattice(dev_channel(ch)) as lat:
+            lat.set_tile(cxk, wpg)
+            lat.init_equilibrium()
+            lat.step(13, "panel")
+            lo, to, po = lat.download()
+        assert_same(lo, lref, f"panel chunk {cxk} wpg {wpg}")
+        assert_same(to, tref, f"panel chunk {cxk} wpg {wpg} tmp")
+        assert np.array_equal(po[2:260], pref[2:260])
+
+
 @pytest.mark.parametrize("variant,tiles", [
     ("panel", [(0, 0), (3, 5), (7, 254), (130, 17), (4, 1)]),
     ("tight_block", [(0, 0), (3, 30), (5, 62), (20, 126), (50, 62)]),

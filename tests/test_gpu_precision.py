@@ -101,15 +101,16 @@ def test_tight_block_reduced_is_refused():
 
 
 @pytest.mark.parametrize("storage", ["f32", "f16"])
-@pytest.mark.parametrize("shape", [(97, 41, 0), (300, 140, 5), (64, 200, 33), (5, 7, 3)])
+@pytest.mark.parametrize("shape", [(97, 41, 0, 0), (300, 140, 5, 0), (64, 200, 33, 0),
+                                   (5, 7, 3, 0), (300, 260, 16, 8), (301, 260, 17, 16)])
 def test_panel_reduced_equals_single_steps(storage, shape):
     """Two steps per launch with the first step in wave registers give the
     same bits as two single steps, for reduced storage too (the first step's
     values are rounded to the storage format in registers)."""
-    lx, ly, cxk = shape
+    lx, ly, cxk, wpg = shape
     ch = L.Channel(lx, ly, lx // 5, ly // 2, max(ly // 10, 1))
     with L.Lattice(ch, storage=storage) as a, L.Lattice(ch, storage=storage) as b:
-        b.set_tile(cxk, 0)
+        b.set_tile(cxk, wpg)
         a.init_equilibrium(); b.init_equilibrium()
         a.step(31, "combine"); b.step(31, "panel")
         la, ta, pa = a.download(); lb, tb, pb = b.download()
