@@ -104,10 +104,8 @@ def measured_traffic(variant):
     m = re.search(r"traffic_bytes_per_launch:\s*(.*)", text)
     if not m:
         return None
-    table = {}
-    for item in m.group(1).split(", "):
-        k, _, v = item.strip().rpartition("=")
-        table[k] = float(v)
+    # names hold ", " themselves (template arguments): split at "=<digits>"
+    table = {k.strip(): float(v) for k, v in re.findall(r"([^=]+)=(\d+)(?:,\s*)?", m.group(1))}
     names = TRAFFIC_KERNELS.get(variant, (KERNEL_OF[variant],))
     if not all(n in table for n in names):
         return None

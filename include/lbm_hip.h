@@ -176,10 +176,11 @@ int64_t lbm_hip_steps_done(const lbm_ctx *ctx);
 int lbm_hip_sync(lbm_ctx *ctx);
 
 /* Time nsteps of a variant with HIP events on the context's stream (work is
- * enqueued exactly as lbm_hip_step does).  ms_total receives the elapsed
- * milliseconds; kernel_ms (may be NULL) receives the average duration of one
- * launch of the variant's dominant kernel, measured with events around a
- * separate batch of launches of that kernel alone. */
+ * enqueued exactly as lbm_hip_step does, including the panel sweep's
+ * perimeter kernel on its second stream, which is joined back before the
+ * closing event).  ms_total receives the elapsed milliseconds.  No
+ * counterpart in the reference (its driver times with gettimeofday,
+ * unsteady.c:387,558). */
 int lbm_hip_time_steps(lbm_ctx *ctx, int variant, int nsteps, double *ms_total);
 
 /* Device bytes one step of a variant moves by design (populations read plus

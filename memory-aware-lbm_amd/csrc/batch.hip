@@ -37,7 +37,8 @@ template <typename S, bool PULL>
 __global__ __launch_bounds__(256) void k_batch(const BatchEntry *__restrict__ ents,
                                                const int *__restrict__ item_chan, int64_t nitems,
                                                int64_t per_wg, const S *__restrict__ A,
-                                               S *__restrict__ B) {
+                                               S *__restrict__ B, const uint32_t *__restrict__ stepctr) {
+    const uint32_t step = sizeof(S) == 4 ? *stepctr : 0u;
     int64_t it = (int64_t)blockIdx.x * per_wg;
     int64_t end = it + per_wg < nitems ? it + per_wg : nitems;
     for (; it < end; ++it) {
@@ -53,20 +54,21 @@ __global__ __launch_bounds__(256) void k_batch(const BatchEntry *__restrict__ en
         if constexpr (sizeof(S) == 8)
             node_step_f64<PULL>(e.p, x, y, (const double *)A + e.off, (double *)B + e.off);
         else
-            node_step_lp<S, PULL>(e.p, x, y, A + e.off, B + e.off);
+            node_step_lp<S, PULL, false>(e.p, x, y, A + e.off, B + e.off, step);
     }
 }
 
 template <typename S>
 static hipError_t batch_launch(bool pull, int nwg, const BatchEntry *ents, const int *item_chan,
-                               int64_t nitems, const void *A, void *B, hipStream_t st) {
+                               int64_t nitems, const void *A, void *B, const uint32_t *step,
+                               hipStream_t st) {
     int64_t per_wg = (nitems + nwg - 1) / nwg;
     if (pull)
         k_batch<S, true><<<nwg, 256, 0, st>>>(ents, item_chan, nitems, per_wg, (const S *)A,
-                                             (S *)B);
+                                             (S *)B, step);
     else
         k_batch<S, false><<<nwg, 256, 0, st>>>(ents, item_chan, nitems, per_wg, (const S *)A,
-                                              (S *)B);
+                                              (S *)B, step);
     return hipGetLastError();
 }
 
@@ -92,6 +94,7 @@ struct lbm_batch {
     int cur = 0;
     int collided = 0;
     int64_t steps = 0;
+    uint32_t *d_step = nullptr;   // collision counter (keys FP32 stochastic rounding)
     hipStream_t stream = nullptr;
     int num_cu = 256;
     int wgs_per_cu = 8;
@@ -148,17 +151,21 @@ hipError_t launch_step(lbm_batch *b, bool pull) {
     int nwg = std::max(1, b->num_cu * b->wgs_per_cu);
     const void *src = b->pool[b->cur];
     void *dst = b->pool[1 - b->cur];
+    hipError_t e;
     switch (b->storage) {
     case LBM_STORE_F64:
         return batch_launch<double>(pull, nwg, b->d_ents, b->d_item_chan, b->nitems, src, dst,
-                                    b->stream);
+                                    b->d_step, b->stream);
     case LBM_STORE_F32:
-        return batch_launch<float>(pull, nwg, b->d_ents, b->d_item_chan, b->nitems, src, dst,
-                                   b->stream);
+        e = batch_launch<float>(pull, nwg, b->d_ents, b->d_item_chan, b->nitems, src, dst,
+                                b->d_step, b->stream);
+        break;
     default:
-        return batch_launch<_Float16>(pull, nwg, b->d_ents, b->d_item_chan, b->nitems, src, dst,
-                                      b->stream);
+        e = batch_launch<_Float16>(pull, nwg, b->d_ents, b->d_item_chan, b->nitems, src, dst,
+                                   b->d_step, b->stream);
+        break;
     }
+    return e == hipSuccess ? launch_bump(b->d_step, 1, b->stream) : e;
 }
 
 int eager_steps(lbm_batch *b, int nsteps) {
@@ -279,6 +286,10 @@ int lbm_hip_batch_create(const lbm_problem *probs, int n, lbm_batch **out) {
     if (e != hipSuccess) return bail(e, "hipMalloc params");
     e = hipMalloc((void **)&b->d_ents, n * sizeof(BatchEntry));
     if (e != hipSuccess) return bail(e, "hipMalloc entries");
+    e = hipMalloc((void **)&b->d_step, sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(b->d_step, 0, sizeof(uint32_t));
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) return bail(e, "step counter");
     // per-channel side parameters
     std::vector<double> host(poff, 0.0);
     for (int c = 0; c < n; ++c) {
@@ -303,6 +314,7 @@ int lbm_hip_batch_create(const lbm_problem *probs, int n, lbm_batch **out) {
             q += 2 * (int64_t)len;
         }
         ent.p.outlet_rho = b->params + q;
+        ent.p.step = b->d_step;
     }
     e = hipMemcpy(b->params, host.data(), (size_t)poff * sizeof(double), hipMemcpyHostToDevice);
     if (e != hipSuccess) return bail(e, "params upload");
@@ -320,6 +332,7 @@ int lbm_hip_batch_destroy(lbm_batch *b) {
     if (b->params) (void)hipFree(b->params);
     if (b->d_ents) (void)hipFree(b->d_ents);
     if (b->d_item_chan) (void)hipFree(b->d_item_chan);
+    if (b->d_step) (void)hipFree(b->d_step);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return LBM_OK;
@@ -327,6 +340,7 @@ int lbm_hip_batch_destroy(lbm_batch *b) {
 
 int lbm_hip_batch_init_equilibrium(lbm_batch *b, const double *const *init_ux) {
     if (!b || !init_ux) return fail(LBM_ERR_ARG, "null argument");
+    HIPCHK(hipMemsetAsync(b->d_step, 0, sizeof(uint32_t), b->stream));
     // both pools to f = 0 (zero bits in FP64, -w in deviation storage)
     for (int k = 0; k < 2; ++k) {
         if (b->storage == LBM_STORE_F64) {

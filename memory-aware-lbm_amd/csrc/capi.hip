@@ -37,6 +37,7 @@ struct lbm_ctx {
     int tile_x, tile_y;
     size_t footprint;
     int graph_steps;               // fused steps per captured graph (0: eager)
+    uint32_t *d_step;              // collision counter (keys FP32 stochastic rounding)
     hipStream_t side;              // second stream for the panel's edge waves
     hipEvent_t fork, join;
     hipGraphExec_t graph[2];       // by starting buffer
@@ -88,7 +89,8 @@ double *dbuf(lbm_ctx *c, int k) { return static_cast<double *>(c->buf[k]); }
 hipError_t run_collide(lbm_ctx *c, bool pull, int from, int to) {
     if (c->storage == LBM_STORE_F64)
         return launch_collide_f64(c->prob, pull, dbuf(c, from), dbuf(c, to), c->stream);
-    return launch_collide_lp(c->storage, c->prob, pull, c->buf[from], c->buf[to], c->stream);
+    hipError_t e = launch_collide_lp(c->storage, c->prob, pull, c->buf[from], c->buf[to], c->stream);
+    return e == hipSuccess ? launch_bump(c->d_step, 1, c->stream) : e;
 }
 
 hipError_t run_stream(lbm_ctx *c, int from, int to) {
@@ -205,6 +207,7 @@ int step_impl(lbm_ctx *c, int variant, int nsteps) {
                 WaveGrid g = wave_grid(c->prob.lay, c->tile_x, c->tile_y);
                 HIPCHK(launch_wpanel(c->storage, c->prob, g, c->buf[c->cur], c->buf[1 - c->cur],
                                      c->stream, c->side, c->fork, c->join));
+                if (c->storage != LBM_STORE_F64) HIPCHK(launch_bump(c->d_step, 2, c->stream));
             } else {
                 BlockGrid g = block_grid(c->prob.lay, c->tile_x,
                                          c->tile_y > 0 ? c->tile_y + 2 : 0);
@@ -313,6 +316,10 @@ int lbm_hip_create(const lbm_problem *p, lbm_ctx **out) {
     if (e != hipSuccess) return cleanup(e, "hipMalloc partials");
     e = hipMalloc((void **)&c->d_out4, 4 * sizeof(double));
     if (e != hipSuccess) return cleanup(e, "hipMalloc out");
+    e = hipMalloc((void **)&c->d_step, sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(c->d_step, 0, sizeof(uint32_t));
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) return cleanup(e, "step counter");
     for (int s = 0; s < 4; ++s) {
         rc = upload_side(c, s, p, p->side[s].p0, p->side[s].p1);
         if (rc) { std::string m = lbm_g_err; lbm_hip_destroy(c); return fail(rc, m); }
@@ -322,6 +329,7 @@ int lbm_hip_create(const lbm_problem *p, lbm_ctx **out) {
         c->prob.side[s].p1 = c->d_p1[s];
     }
     c->prob.outlet_rho = c->d_outlet;
+    c->prob.step = c->d_step;
     c->cur = 0;
     c->collided = 0;
     c->steps = 0;
@@ -342,6 +350,7 @@ int lbm_hip_destroy(lbm_ctx *c) {
     if (c->d_outlet) (void)hipFree(c->d_outlet);
     if (c->d_partial) (void)hipFree(c->d_partial);
     if (c->d_out4) (void)hipFree(c->d_out4);
+    if (c->d_step) (void)hipFree(c->d_step);
     if (c->side) (void)hipStreamSynchronize(c->side);
     if (c->fork) (void)hipEventDestroy(c->fork);
     if (c->join) (void)hipEventDestroy(c->join);
@@ -367,6 +376,7 @@ int lbm_hip_init_equilibrium(lbm_ctx *c, const double *init_ux) {
     if (e == hipSuccess)
         e = c->storage == LBM_STORE_F64 ? launch_init_f64(L, d_ux, dbuf(c, 0), c->stream)
                                         : launch_init_lp(c->storage, L, d_ux, c->buf[0], c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->d_step, 0, sizeof(uint32_t), c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(d_ux);
     if (e != hipSuccess) return hip_fail(e, "init_equilibrium");
@@ -383,6 +393,7 @@ int lbm_hip_upload(lbm_ctx *c, const double *lattice, const double *tmp,
     int rc = transfer(c, const_cast<double *>(lattice), 0, true);
     if (!rc) rc = transfer(c, const_cast<double *>(tmp), 1, true);
     if (rc) return rc;
+    HIPCHK(hipMemsetAsync(c->d_step, 0, sizeof(uint32_t), c->stream));
     if (outlet_rho) {
         HIPCHK(hipMemcpy(c->d_outlet, outlet_rho, (c->prob.lay.ly + 2) * sizeof(double),
                          hipMemcpyHostToDevice));

@@ -165,6 +165,17 @@ hipError_t launch_diag_f64(const Problem &p, bool pull, const double *src, doubl
     return hipGetLastError();
 }
 
+// collision counter of a context or batch: advanced by its own one-thread
+// kernel on the stream (so captured graphs advance it on every replay)
+__global__ void k_bump(uint32_t *ctr, uint32_t by) {
+    if (threadIdx.x == 0) *ctr += by;
+}
+
+hipError_t launch_bump(uint32_t *ctr, uint32_t by, hipStream_t st) {
+    k_bump<<<1, 64, 0, st>>>(ctr, by);
+    return hipGetLastError();
+}
+
 hipError_t launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t st) {
     int64_t n = bytes / 16;
     int blocks = (int)((n + 255) / 256 < 256 * 1024 ? (n + 255) / 256 : 256 * 1024);

@@ -29,7 +29,9 @@ __global__ __launch_bounds__(256) void k_collide_lp(Problem p, const S *__restri
     int y = blockIdx.x * blockDim.x + threadIdx.x + 1;
     int x = blockIdx.y + 1;
     if (y > p.lay.ly) return;
-    node_step_lp<S, PULL>(p, x, y, src, dst);
+    const uint32_t step = sizeof(S) == 4 ? *p.step : 0u;
+    if constexpr (sizeof(S) == 4) fp32_round_toward_zero();
+    node_step_lp<S, PULL, true>(p, x, y, src, dst, step);
 }
 
 template <typename S>

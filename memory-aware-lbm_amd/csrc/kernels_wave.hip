@@ -75,22 +75,27 @@ __device__ __forceinline__ double pulled_rho(const Layout &L, const S *__restric
     return rho_of(f);
 }
 
-// One node's collision on compute-type populations h, side s (-1: interior).
+// One node's collision on compute-type populations h, side s (-1: interior);
+// `step` is the collision's index (it keys FP32's stochastic rounding, so the
+// first step held in registers rounds exactly as a stored single step would).
 template <typename S>
 __device__ __forceinline__ void collide_any(const Problem &p, int s, int x, int y,
-                                            typename Calc<S>::T h[9], double right_p0) {
+                                            typename Calc<S>::T h[9], double right_p0,
+                                            uint32_t step) {
+    // the panel kernels run FP32 storage with toward-zero FP32 rounding
     if constexpr (sizeof(S) == 8) {
         collide_class(p, s, x, y, h, right_p0);
     } else {
+        const SrSeed hs = sizeof(S) == 4 ? sr_seed(p.lay.node(x, y), step) : SrSeed{0u, 0u};
         if (s < 0) {
             if (p.r2 >= 0 && in_disc(p, x, y)) bounce_dev(h);
-            else bgk_dev(h, (float)p.omega);
+            else bgk_lp<S, true>(h, p.omega, hs);
             return;
         }
         double f[9];
         for_pops([&]<int I>() { f[I] = wgt(I) + (double)h[I]; });
         collide_class(p, s, x, y, f, right_p0);
-        for_pops([&]<int I>() { h[I] = (float)(f[I] - wgt(I)); });
+        for_pops([&]<int I>() { h[I] = to_dev<S, I, true>(f[I], hs); });
     }
 }
 
@@ -128,6 +133,7 @@ __global__ __launch_bounds__(64) void k_wpanel_edge(Problem p, WaveGrid g,
     const bool row_in = y >= 1 && y <= L.ly;
     const bool owner = lane >= 1 && lane <= WAVE_ROWS && y <= L.ly;
     const int s_row = (y == 1) ? LOWER : (y == L.ly) ? UPPER : -1;
+    const uint32_t m = sizeof(S) == 4 ? *p.step : 0u;   // index of the first collision
 
     T cur[9], c1[9], c2[3];      // columns x, x-1 and (1, 5, 8 of) x-2
     S fn[9];                     // next column's pulled populations
@@ -140,6 +146,7 @@ __global__ __launch_bounds__(64) void k_wpanel_edge(Problem p, WaveGrid g,
         for_pops([&]<int I>() { f[I] = P[I * L.plane + n - cx(I) * L.pitch - cy(I)]; });
     };
     if (row_in && X0 - 1 >= 1) pull(X0 - 1, fn);
+    if constexpr (sizeof(S) == 4) fp32_round_toward_zero();
 
     for (int x = X0 - 1; x <= X1; ++x) {
         // ---- step m, column x, all 64 rows -> registers
@@ -150,7 +157,7 @@ __global__ __launch_bounds__(64) void k_wpanel_edge(Problem p, WaveGrid g,
             double rp0 = 0.;
             if (s == RIGHT && p.zgb)
                 rp0 = zgb_rho(pulled_rho<S>(L, P, L.lx - 1, y), pulled_rho<S>(L, P, L.lx - 2, y));
-            collide_any<S>(p, s, x, y, cur, rp0);
+            collide_any<S>(p, s, x, y, cur, rp0, m);
             for_pops([&]<int I>() { cur[I] = stored<S>(cur[I]); });
         }
         // ---- neighbour rows' first-step values (all lanes take part)
@@ -182,7 +189,7 @@ __global__ __launch_bounds__(64) void k_wpanel_edge(Problem p, WaveGrid g,
                 rp0 = zgb_rho(rho1, rho2);
                 p.outlet_rho[y] = p.zgb ? rp0 : p.side[RIGHT].p0[y];
             }
-            collide_any<S>(p, s, xo, y, h, rp0);
+            collide_any<S>(p, s, xo, y, h, rp0, m + 1);
             for_pops([&]<int I>() {
                 if constexpr (sizeof(S) == 8) __builtin_nontemporal_store(h[I], &Q[I * L.plane + n]);
                 else __builtin_nontemporal_store(narrow<S>(h[I]), &Q[I * L.plane + n]);
@@ -253,6 +260,7 @@ __global__ __launch_bounds__(64 * WPG) void k_wpanel_in(Problem p, WaveGrid g,
     const int nbi = g.b_hi - g.b_lo;
     const WaveSpan w = wave_span(L, g, g.b_lo + t % nbi, g.c_lo + t / nbi);
     const int y = w.Y0 - 1 + lane;
+    const uint32_t m = sizeof(S) == 4 ? *p.step : 0u;   // index of the first collision
     const uint32_t colb = (uint32_t)L.pitch * (uint32_t)sizeof(S);
     const uint32_t lane_off =
         (lane >= 1 && lane <= WAVE_ROWS) ? (uint32_t)(y + L.yoff) * (uint32_t)sizeof(S) : DROP;
@@ -275,37 +283,66 @@ __global__ __launch_bounds__(64 * WPG) void k_wpanel_in(Problem p, WaveGrid g,
         const int64_t n = L.node(x, y);
         for_pops([&]<int I>() { dst[I] = P[I * L.plane + n - cx(I) * L.pitch - cy(I)]; });
     };
-    // iteration x: step m of column x, step m+1 of column x-1 (stored when
-    // x-1 >= X0); LOAD pulls column x + DEPTH into the buffer just consumed
-    auto iter = [&]<bool LOAD>(int x, S (&buf)[9]) {
-        for_pops([&]<int I>() { cur[I] = as_calc<S>(buf[I]); });
-        if constexpr (LOAD) pull(x + DEPTH, buf);
-        collide_any<S>(p, -1, x, y, cur, 0.);
-        for_pops([&]<int I>() { cur[I] = stored<S>(cur[I]); });
-        const T up2 = lane_up(c1[2]), up5 = lane_up(c2[1]), up6 = lane_up(cur[6]);
-        const T dn4 = lane_down(c1[4]), dn7 = lane_down(cur[7]), dn8 = lane_down(c2[2]);
-        T h[9] = {c1[0], c2[0], up2, cur[3], dn4, up5, up6, dn7, dn8};
-        collide_any<S>(p, -1, x - 1, y, h, 0.);
-        const uint32_t off = x - 1 >= w.X0 ? lane_off + (uint32_t)(x - 1 - w.X0) * colb : DROP;
-        for_pops([&]<int I>() { buffer_store<S>(to_storage<S>(h[I]), rs[I], off); });
-        c2[0] = c1[1];
-        c2[1] = c1[5];
-        c2[2] = c1[8];
-        for_pops([&]<int I>() { c1[I] = cur[I]; });
+    // Iteration x: step m of column x, step m+1 of column x-1, stores of
+    // column x-1; LOAD pulls column x + DEPTH into the buffer just consumed.
+    // Kinds: FILL iterations only put loads in flight, HEAD iterations compute
+    // the first step of columns X0-1 and X0, FULL iterations do both steps,
+    // TAIL iterations are the last DEPTH, with nothing left to load.  Every
+    // kind issues the same stores (FILL and HEAD ones dropped), so the memory
+    // traffic of the loop and of the code before it looks alike to the
+    // compiler's wait-count analysis.
+    enum { FILL, HEAD, FULL, TAIL };
+    auto iter = [&]<int KIND>(int x, S (&buf)[9]) {
+        if constexpr (KIND == FILL) {
+            pull(x + DEPTH, buf);
+            for_pops([&]<int I>() { buffer_store<S>(S(0), rs[I], DROP); });
+        } else {
+            for_pops([&]<int I>() { cur[I] = as_calc<S>(buf[I]); });
+            if constexpr (KIND != TAIL) pull(x + DEPTH, buf);
+            collide_any<S>(p, -1, x, y, cur, 0., m);
+            for_pops([&]<int I>() { cur[I] = stored<S>(cur[I]); });
+            if constexpr (KIND == HEAD) {
+                for_pops([&]<int I>() { buffer_store<S>(S(0), rs[I], DROP); });
+            } else {
+                const T up2 = lane_up(c1[2]), up5 = lane_up(c2[1]), up6 = lane_up(cur[6]);
+                const T dn4 = lane_down(c1[4]), dn7 = lane_down(cur[7]), dn8 = lane_down(c2[2]);
+                T h[9] = {c1[0], c2[0], up2, cur[3], dn4, up5, up6, dn7, dn8};
+                collide_any<S>(p, -1, x - 1, y, h, 0., m + 1);
+                const uint32_t off = lane_off + (uint32_t)(x - 1 - w.X0) * colb;
+                for_pops([&]<int I>() { buffer_store<S>(to_storage<S>(h[I]), rs[I], off); });
+            }
+            c2[0] = c1[1];
+            c2[1] = c1[5];
+            c2[2] = c1[8];
+            for_pops([&]<int I>() { c1[I] = cur[I]; });
+        }
         if constexpr (WPG > 4) __builtin_amdgcn_s_barrier();
     };
 
-    // iterations X0-1-DEPTH .. X1-DEPTH load; the first DEPTH of them run on
-    // zeros and only fill the pipeline (their stores are dropped, and their
-    // values leave the registers before column X0's second step reads them)
+    // x runs X0-1-DEPTH .. X1: DEPTH fill iterations, HEAD at X0-1 and X0,
+    // FULL up to X1-DEPTH, then DEPTH tail iterations
     int x = w.X0 - 1 - DEPTH;
-    const int nload = (w.X1 - w.X0 + 2) / DEPTH;
-    for (int k = 0; k < nload; ++k, x += DEPTH) {
-        iter.template operator()<true>(x, f[0]);
-        if constexpr (DEPTH == 2) iter.template operator()<true>(x + 1, f[1]);
+    if constexpr (sizeof(S) == 4) fp32_round_toward_zero();
+    if constexpr (DEPTH == 2) {
+        iter.template operator()<FILL>(x, f[0]);
+        iter.template operator()<FILL>(x + 1, f[1]);
+        iter.template operator()<HEAD>(x + 2, f[0]);
+        iter.template operator()<HEAD>(x + 3, f[1]);
+        x += 4;
+        for (int k = 0; k < (w.X1 - w.X0) / 2 - 1; ++k, x += 2) {
+            iter.template operator()<FULL>(x, f[0]);
+            iter.template operator()<FULL>(x + 1, f[1]);
+        }
+        iter.template operator()<TAIL>(x, f[0]);
+        iter.template operator()<TAIL>(x + 1, f[1]);
+    } else {
+        iter.template operator()<FILL>(x, f[0]);
+        iter.template operator()<HEAD>(x + 1, f[0]);
+        iter.template operator()<HEAD>(x + 2, f[0]);
+        x += 3;
+        for (; x < w.X1; ++x) iter.template operator()<FULL>(x, f[0]);
+        iter.template operator()<TAIL>(x, f[0]);
     }
-    iter.template operator()<false>(x, f[0]);
-    if constexpr (DEPTH == 2) iter.template operator()<false>(x + 1, f[1]);
 }
 
 // Chunk width 16 and 8 lockstep waves per workgroup unless given.  Short
@@ -316,6 +353,9 @@ __global__ __launch_bounds__(64 * WPG) void k_wpanel_in(Problem p, WaveGrid g,
 WaveGrid wave_grid(const Layout &L, int cx, int wpg) {
     WaveGrid g;
     g.cx = cx >= 3 ? cx : 16;
+    // the interior kernel's buffer resources span one chunk of one plane and
+    // must stay below the drop offset (2 GiB), at up to 8 bytes per element
+    while (g.cx > 3 && (int64_t)g.cx * L.pitch * 8 >= (int64_t)DROP) g.cx /= 2;
     g.wpg = (wpg == 4 || wpg == 16) ? wpg : 8;
     g.nbands = (L.ly + WAVE_ROWS - 1) / WAVE_ROWS;
     g.ncx = (L.lx + g.cx - 1) / g.cx;

@@ -62,6 +62,9 @@ struct Problem {
     SideBC side[4];
     int zgb;             // right side density by extrapolation
     double *outlet_rho;  // device, ly+2: outlet density used by the last step
+    // device counter of the collisions done so far (reduced storage only: it
+    // keys the stochastic rounding); a launch reads it, the next one bumps it
+    const uint32_t *step;
 };
 
 __host__ __device__ inline bool in_disc(const Problem &p, int x, int y) {

@@ -93,9 +93,10 @@ __device__ __forceinline__ double density_lp(const Layout &L, const S *__restric
     return rho_of(f);
 }
 
-template <typename S, bool PULL>
+template <typename S, bool PULL, bool RZ>
 __device__ __forceinline__ void perimeter_lp(const Problem &p, int s, int x, int y, float g[9],
-                                          const S *__restrict__ src, S *__restrict__ dst) {
+                                          const S *__restrict__ src, S *__restrict__ dst,
+                                          SrSeed h) {
     const Layout &L = p.lay;
     int64_t n = L.node(x, y);
     for_pops([&]<int I>() {
@@ -112,22 +113,26 @@ __device__ __forceinline__ void perimeter_lp(const Problem &p, int s, int x, int
         rp0 = zgb_rho(density_lp<S, PULL>(L, src, L.lx - 1, y), density_lp<S, PULL>(L, src, L.lx - 2, y));
     if (s == RIGHT) p.outlet_rho[y] = p.zgb ? rp0 : p.side[RIGHT].p0[y];
     collide_class(p, s, x, y, f, rp0);
-    for_pops([&]<int I>() { g[I] = (float)(f[I] - wgt(I)); });
+    for_pops([&]<int I>() { g[I] = to_dev<S, I, RZ>(f[I], h); });
 }
 
-template <typename S, bool PULL>
+// One collision of node (x, y) with reduced storage; `step` is the index of
+// this collision (it keys FP32's stochastic rounding); RZ as in sr_f32.
+template <typename S, bool PULL, bool RZ>
 __device__ __forceinline__ void node_step_lp(const Problem &p, int x, int y,
-                                             const S *__restrict__ src, S *__restrict__ dst) {
+                                             const S *__restrict__ src, S *__restrict__ dst,
+                                             uint32_t step) {
     const Layout &L = p.lay;
     int64_t n = L.node(x, y);
     float g[9];
     for_pops([&]<int I>() {
         g[I] = PULL ? ld(&src[I * L.plane + n - cx(I) * L.pitch - cy(I)]) : ld(&src[I * L.plane + n]);
     });
+    const SrSeed h = sizeof(S) == 4 ? sr_seed(n, step) : SrSeed{0u, 0u};
     int s = side_of(p, x, y);
-    if (s >= 0) perimeter_lp<S, PULL>(p, s, x, y, g, src, dst);
+    if (s >= 0) perimeter_lp<S, PULL, RZ>(p, s, x, y, g, src, dst, h);
     else if (p.r2 >= 0 && in_disc(p, x, y)) bounce_dev(g);
-    else bgk_dev(g, (float)p.omega);
+    else bgk_lp<S, RZ>(g, p.omega, h);
     for_pops([&]<int I>() { st_nt(narrow<S>(g[I]), &dst[I * L.plane + n]); });
 }
 

@@ -1,0 +1,34 @@
+"""bench.py's own plumbing, on the CPU: the HBM traffic it reports is read
+from the committed rocprofv3 summary, and runs on more than one GPU are
+refused with a message."""
+import importlib.util
+import json
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def load_bench():
+    spec = importlib.util.spec_from_file_location("bench", os.path.join(ROOT, "bench.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def test_panel_traffic_from_profile():
+    b = load_bench()
+    t = b.measured_traffic("panel")
+    # interior + perimeter kernels of one two-step launch on 16384 x 4096:
+    # at least the algorithmic 144 B per node, at most 1.5x that
+    alg = 144.0 * 16384 * 4096
+    assert t is not None and alg <= t <= 1.5 * alg, t
+    assert b.measured_traffic("tight_block") > alg
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
