@@ -45,9 +45,9 @@ enum {
  *               (collide_tight, lb.c:442-931)
  *   TIGHT_BLOCK two time steps per launch in LDS-resident 2-D tiles
  *               (collide_tight_block, lb.c:936-1289)
- *   PANEL       two time steps per launch, each workgroup marching a panel
- *               with a rolling LDS window (collide_tight_panel_iy,
- *               lb.c:1402-1779)
+ *   PANEL       two time steps per launch, each wave64 marching a band of
+ *               rows with the first step held in registers
+ *               (collide_tight_panel_iy, lb.c:1402-1779)
  * All five give bit-identical FP64 results, as the reference's serial ones do. */
 typedef enum {
     LBM_ORIGIN = 0,
@@ -58,8 +58,10 @@ typedef enum {
 } lbm_variant;
 
 /* Population storage.  F64 is the reference's arithmetic.  F32/F16 store the
- * deviations f_i - w_i and collide in FP32, keeping FP64 for the wall
- * operators' moments and for reductions (no reference counterpart). */
+ * deviations f_i - w_i.  F32 collides in FP64 and rounds each stored value
+ * stochastically (keyed by node, collision index and population, so runs are
+ * deterministic); F16 collides in FP32 and rounds to nearest.  Wall operators
+ * and reductions run in FP64 for both.  No reference counterpart. */
 typedef enum { LBM_STORE_F64 = 0, LBM_STORE_F32 = 1, LBM_STORE_F16 = 2 } lbm_storage;
 
 /* Straight-wall operator families and kinds (boundaries.h:72-102). */
@@ -137,8 +139,7 @@ int lbm_hip_propagate(lbm_ctx *ctx);
 
 /* Block-size knobs of the blocked variants (the reference's blk_size /
  * thread_block, unsteady.c:93,258).  TIGHT_BLOCK: tile extent in x and y.
- * PANEL: tile_x = columns per wave's march (>= 3), tile_y = waves per
- * workgroup marching in lockstep (4, 8 or 16; other values pick 8).
+ * PANEL: tile_x = columns per wave's march (>= 3); tile_y is not used.
  * 0 = default.  The results do not depend on either knob. */
 int lbm_hip_set_tile(lbm_ctx *ctx, int tile_x, int tile_y);
 

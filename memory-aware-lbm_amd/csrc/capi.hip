@@ -204,7 +204,7 @@ int step_impl(lbm_ctx *c, int variant, int nsteps) {
         }
         for (; k + 2 <= nsteps; k += 2) {
             if (variant == LBM_PANEL) {
-                WaveGrid g = wave_grid(c->prob.lay, c->tile_x, c->tile_y);
+                WaveGrid g = wave_grid(c->prob.lay, c->tile_x);
                 HIPCHK(launch_wpanel(c->storage, c->prob, g, c->buf[c->cur], c->buf[1 - c->cur],
                                      c->stream, c->side, c->fork, c->join));
                 if (c->storage != LBM_STORE_F64) HIPCHK(launch_bump(c->d_step, 2, c->stream));

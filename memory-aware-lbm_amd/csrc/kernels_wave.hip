@@ -28,6 +28,8 @@
 // collisions, FP64 walls).  For reduced storage the first step's populations
 // are rounded to S exactly as a store would round them, so two steps here
 // equal two single steps bit for bit.
+#include <utility>
+
 #include "launch.hpp"
 #include "fields.hpp"
 #include "node_step.hpp"
@@ -243,11 +245,20 @@ __device__ __forceinline__ int xcd_block(int b, int nblk) {
     return xcd < rem ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
 }
 
-// DEPTH = columns whose loads are in flight; (cx + 2) % DEPTH == 0.  WPG
-// waves per workgroup take consecutive bands; above 4 they march in lockstep
-// (one barrier per column), so the workgroup's loads of a column arrive at
-// the memory together as one contiguous run per population.
-template <typename S, int DEPTH, int WPG>
+// Compile-time loop over K = 0 .. N-1.
+template <int N, typename F> __device__ __forceinline__ void static_for(F &&f) {
+    [&]<int... K>(std::integer_sequence<int, K...>) {
+        (f.template operator()<K>(), ...);
+    }(std::make_integer_sequence<int, N>{});
+}
+
+// WPG waves per workgroup take consecutive bands and march in lockstep (one
+// barrier per column), so the workgroup's loads of a column arrive at the
+// memory together as one contiguous run per population.
+constexpr int WPG = 8;
+
+// DEPTH = columns whose loads are in flight (1 to 4); cx % DEPTH == 0.
+template <typename S, int DEPTH>
 __global__ __launch_bounds__(64 * WPG) void k_wpanel_in(Problem p, WaveGrid g,
                                                         const S *__restrict__ P,
                                                         S *__restrict__ Q) {
@@ -255,7 +266,7 @@ __global__ __launch_bounds__(64 * WPG) void k_wpanel_in(Problem p, WaveGrid g,
     const Layout &L = p.lay;
     const int lane = threadIdx.x & 63;
     const int t = __builtin_amdgcn_readfirstlane(
-        (WPG > 4 ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x) * WPG + (threadIdx.x >> 6));
+        xcd_block(blockIdx.x, gridDim.x) * WPG + (threadIdx.x >> 6));
     if (t >= g.nin) return;                              // whole waves only
     const int nbi = g.b_hi - g.b_lo;
     const WaveSpan w = wave_span(L, g, g.b_lo + t % nbi, g.c_lo + t / nbi);
@@ -279,9 +290,20 @@ __global__ __launch_bounds__(64 * WPG) void k_wpanel_in(Problem p, WaveGrid g,
     for (int i = 0; i < 9; ++i) { cur[i] = 0; c1[i] = 0; }
     c2[0] = c2[1] = c2[2] = 0;
 
+    // per population, a wave-uniform base (SGPRs) with pb[I][v(x)] the value
+    // pulled into (x, y), v(x) = (x - X0 + 3) * pitch + y + yoff: one 32-bit
+    // lane offset per column serves all nine loads
+    const S *pb[9];
+    for_pops([&]<int I>() {
+        pb[I] = P + I * L.plane + (int64_t)(w.X0 - 3 - cx(I)) * L.pitch - cy(I);
+    });
+    const uint32_t vy = (uint32_t)(y + L.yoff) * (uint32_t)sizeof(S);
     auto pull = [&](int x, S (&dst)[9]) {
-        const int64_t n = L.node(x, y);
-        for_pops([&]<int I>() { dst[I] = P[I * L.plane + n - cx(I) * L.pitch - cy(I)]; });
+        // byte offset kept 32-bit so the loads take the SGPR-base form
+        const uint32_t v = (uint32_t)(x - w.X0 + 3) * colb + vy;
+        for_pops([&]<int I>() {
+            dst[I] = *reinterpret_cast<const S *>(reinterpret_cast<const char *>(pb[I]) + v);
+        });
     };
     // Iteration x: step m of column x, step m+1 of column x-1, stores of
     // column x-1; LOAD pulls column x + DEPTH into the buffer just consumed.
@@ -316,47 +338,33 @@ __global__ __launch_bounds__(64 * WPG) void k_wpanel_in(Problem p, WaveGrid g,
             c2[2] = c1[8];
             for_pops([&]<int I>() { c1[I] = cur[I]; });
         }
-        if constexpr (WPG > 4) __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_s_barrier();
     };
 
     // x runs X0-1-DEPTH .. X1: DEPTH fill iterations, HEAD at X0-1 and X0,
-    // FULL up to X1-DEPTH, then DEPTH tail iterations
+    // FULL up to X1-DEPTH, then DEPTH tail iterations; the j-th iteration
+    // (from 0) consumes and reloads buffer j % DEPTH
     int x = w.X0 - 1 - DEPTH;
     if constexpr (sizeof(S) == 4) fp32_round_toward_zero();
-    if constexpr (DEPTH == 2) {
-        iter.template operator()<FILL>(x, f[0]);
-        iter.template operator()<FILL>(x + 1, f[1]);
-        iter.template operator()<HEAD>(x + 2, f[0]);
-        iter.template operator()<HEAD>(x + 3, f[1]);
-        x += 4;
-        for (int k = 0; k < (w.X1 - w.X0) / 2 - 1; ++k, x += 2) {
-            iter.template operator()<FULL>(x, f[0]);
-            iter.template operator()<FULL>(x + 1, f[1]);
-        }
-        iter.template operator()<TAIL>(x, f[0]);
-        iter.template operator()<TAIL>(x + 1, f[1]);
-    } else {
-        iter.template operator()<FILL>(x, f[0]);
-        iter.template operator()<HEAD>(x + 1, f[0]);
-        iter.template operator()<HEAD>(x + 2, f[0]);
-        x += 3;
-        for (; x < w.X1; ++x) iter.template operator()<FULL>(x, f[0]);
-        iter.template operator()<TAIL>(x, f[0]);
-    }
+    static_for<DEPTH>([&]<int D>() { iter.template operator()<FILL>(x + D, f[D]); });
+    x += DEPTH;
+    iter.template operator()<HEAD>(x, f[0]);
+    iter.template operator()<HEAD>(x + 1, f[1 % DEPTH]);
+    x += 2;
+    for (int k = 0; k < (w.X1 - w.X0) / DEPTH - 1; ++k, x += DEPTH)
+        static_for<DEPTH>([&]<int D>() { iter.template operator()<FULL>(x + D, f[(2 + D) % DEPTH]); });
+    static_for<DEPTH>([&]<int D>() { iter.template operator()<TAIL>(x + D, f[(2 + D) % DEPTH]); });
 }
 
-// Chunk width 16 and 8 lockstep waves per workgroup unless given.  Short
-// chunks keep the last round of waves small and the march's pipeline-fill
-// share low enough: (cx + 2) / cx first-step work, (cx + 4) / cx iterations.
-// On 16384 x 4096 (FP64) 16 x 8 gives 59 GLUPS against 52.8 for 16 x 4
-// without lockstep and 55 for 128 x 16.
-WaveGrid wave_grid(const Layout &L, int cx, int wpg) {
+// Chunk width 16 unless given.  Short chunks keep the last round of waves
+// small at (cx + 2) / cx first-step work.  On 16384 x 4096 (FP64) chunks of
+// 16 give 59 GLUPS against 58 for 32, 57 for 64 and 55 for 128.
+WaveGrid wave_grid(const Layout &L, int cx) {
     WaveGrid g;
     g.cx = cx >= 3 ? cx : 16;
     // the interior kernel's buffer resources span one chunk of one plane and
     // must stay below the drop offset (2 GiB), at up to 8 bytes per element
     while (g.cx > 3 && (int64_t)g.cx * L.pitch * 8 >= (int64_t)DROP) g.cx /= 2;
-    g.wpg = (wpg == 4 || wpg == 16) ? wpg : 8;
     g.nbands = (L.ly + WAVE_ROWS - 1) / WAVE_ROWS;
     g.ncx = (L.lx + g.cx - 1) / g.cx;
     g.ntasks = g.nbands * g.ncx;
@@ -383,34 +391,21 @@ WaveGrid wave_grid(const Layout &L, int cx, int wpg) {
 hipError_t launch_wpanel(int storage, const Problem &p, const WaveGrid &g, const void *P, void *Q,
                          hipStream_t st, hipStream_t side, hipEvent_t fork, hipEvent_t join) {
     if (g.cx < 3) return hipErrorInvalidValue;
-    const bool deep = (g.cx + 2) % 2 == 0;
-    const int blocks = (g.nin + g.wpg - 1) / g.wpg;
+    // two columns of loads in flight when the chunk width allows (three or
+    // four measured no faster, for any storage format)
+    const int depth = g.cx % 2 == 0 ? 2 : 1;
+    const int blocks = (g.nin + WPG - 1) / WPG;
     hipError_t e = hipEventRecord(fork, st);
     if (e == hipSuccess) e = hipStreamWaitEvent(side, fork, 0);
     if (e != hipSuccess) return e;
     auto go = [&](auto tag) {
         using S = decltype(tag);
         if (g.nedge) k_wpanel_edge<S><<<g.nedge, 64, 0, side>>>(p, g, (const S *)P, (S *)Q);
-        if (g.nin) {
-            auto in = [&](auto depth, auto wpg) {
-                constexpr int D = decltype(depth)::value, W = decltype(wpg)::value;
-                k_wpanel_in<S, D, W><<<blocks, 64 * W, 0, st>>>(p, g, (const S *)P, (S *)Q);
-            };
-            using I1 = std::integral_constant<int, 1>;
-            using I2 = std::integral_constant<int, 2>;
-            using W4 = std::integral_constant<int, 4>;
-            using W8 = std::integral_constant<int, 8>;
-            using W16 = std::integral_constant<int, 16>;
-            if (deep) {
-                if (g.wpg == 8) in(I2{}, W8{});
-                else if (g.wpg == 16) in(I2{}, W16{});
-                else in(I2{}, W4{});
-            } else {
-                if (g.wpg == 8) in(I1{}, W8{});
-                else if (g.wpg == 16) in(I1{}, W16{});
-                else in(I1{}, W4{});
-            }
-        }
+        if (!g.nin) return;
+        const S *src = (const S *)P;
+        S *dst = (S *)Q;
+        if (depth == 2) k_wpanel_in<S, 2><<<blocks, 64 * WPG, 0, st>>>(p, g, src, dst);
+        else k_wpanel_in<S, 1><<<blocks, 64 * WPG, 0, st>>>(p, g, src, dst);
     };
     if (storage == 0) go(double{});
     else if (storage == 1) go(float{});

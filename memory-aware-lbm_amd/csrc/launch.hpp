@@ -58,7 +58,6 @@ struct WaveGrid {
     int nbands, ncx, ntasks;
     int b_lo, b_hi, c_lo, c_hi;    // (band, chunk) rectangle of the interior waves
     int nin, nedge;                // interior waves, perimeter waves
-    int wpg;                       // interior waves per workgroup (4, 8 or 16)
 };
 
 // Rows [Y0-1, Y0+62] (lanes) and columns [X0, X1) owned by wave (band, ck);
@@ -72,7 +71,7 @@ __host__ __device__ inline WaveSpan wave_span(const Layout &L, const WaveGrid &g
     return w;
 }
 
-WaveGrid wave_grid(const Layout &L, int cx, int wpg);
+WaveGrid wave_grid(const Layout &L, int cx);
 hipError_t launch_wpanel(int storage, const Problem &p, const WaveGrid &g, const void *P, void *Q,
                          hipStream_t st, hipStream_t side, hipEvent_t fork, hipEvent_t join);
 
