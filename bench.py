@@ -40,11 +40,11 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "GLUPS & % of HBM roofline (FP64 vs FP32-store); serving p50/p99 latency"
 KERNEL_OF = {"origin": "k_collide<false>", "combine": "k_collide<true>",
              "tight": "k_collide<true>", "tight_block": "k_block<64>",
-             "panel": "k_wpanel_in<double, 2, 8>"}
+             "panel": "k_wpanel_in<double, 2>"}
 LOWP_KERNEL_OF = {("f32", "combine"): "k_collide_lp<float, true>",
                   ("f16", "combine"): "k_collide_lp<_Float16, true>",
-                  ("f32", "panel"): "k_wpanel_in<float, 2, 8>",
-                  ("f16", "panel"): "k_wpanel_in<_Float16, 2, 8>"}
+                  ("f32", "panel"): "k_wpanel_in<float, 2>",
+                  ("f16", "panel"): "k_wpanel_in<_Float16, 2>"}
 UPDATES_PER_LAUNCH = {"origin": 0.5, "combine": 1, "tight": 1, "tight_block": 2, "panel": 2}
 PROFILE = os.path.join(ROOT, "profiles", "kernels_16384x4096.md")
 
@@ -92,7 +92,7 @@ def time_variant(lbm, ch, variant, steps, warmup, storage="f64"):
 
 
 # kernels of one launch of a variant whose HBM traffic adds up to the launch's
-TRAFFIC_KERNELS = {"panel": ("k_wpanel_in<double, 2, 8>", "k_wpanel_edge<double>")}
+TRAFFIC_KERNELS = {"panel": ("k_wpanel_in<double, 2>", "k_wpanel_edge<double>")}
 
 
 def measured_traffic(variant):

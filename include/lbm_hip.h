@@ -143,9 +143,10 @@ int lbm_hip_propagate(lbm_ctx *ctx);
  * 0 = default.  The results do not depend on either knob. */
 int lbm_hip_set_tile(lbm_ctx *ctx, int tile_x, int tile_y);
 
-/* Capture graph_steps consecutive fused steps (combine/tight variants) into
- * a hipGraph and replay it for long runs; 0 launches every step (default).
- * No counterpart in the reference. */
+/* Capture graph_steps consecutive steps into a hipGraph and replay it for
+ * long runs: fused steps for combine/tight, graph_steps / 2 two-step
+ * launches for panel.  0 launches every step (default).  No counterpart in
+ * the reference. */
 int lbm_hip_set_graph_steps(lbm_ctx *ctx, int graph_steps);
 
 /* 1 if all work enqueued on the context has finished, 0 if some is still

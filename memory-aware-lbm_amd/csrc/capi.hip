@@ -40,7 +40,8 @@ struct lbm_ctx {
     uint32_t *d_step;              // collision counter (keys FP32 stochastic rounding)
     hipStream_t side;              // second stream for the panel's edge waves
     hipEvent_t fork, join;
-    hipGraphExec_t graph[2];       // by starting buffer
+    hipGraphExec_t graph[2];       // fused steps, by starting buffer
+    hipGraphExec_t pgraph[2];      // panel launches, by starting buffer
 };
 
 namespace {
@@ -120,11 +121,45 @@ int to_streamed(lbm_ctx *c) {
 }
 
 void drop_graphs(lbm_ctx *c) {
-    for (int k = 0; k < 2; ++k)
+    for (int k = 0; k < 2; ++k) {
         if (c->graph[k]) {
             (void)hipGraphExecDestroy(c->graph[k]);
             c->graph[k] = nullptr;
         }
+        if (c->pgraph[k]) {
+            (void)hipGraphExecDestroy(c->pgraph[k]);
+            c->pgraph[k] = nullptr;
+        }
+    }
+}
+
+// one two-step panel launch from buffer `from` into 1 - from
+hipError_t panel_launch(lbm_ctx *c, int from) {
+    WaveGrid g = wave_grid(c->prob.lay, c->tile_x);
+    hipError_t e = launch_wpanel(c->storage, c->prob, g, c->buf[from], c->buf[1 - from], c->stream,
+                                 c->side, c->fork, c->join);
+    if (e == hipSuccess && c->storage != LBM_STORE_F64) e = launch_bump(c->d_step, 2, c->stream);
+    return e;
+}
+
+// capture graph_steps / 2 panel launches starting from buffer `start` (the
+// perimeter stream joins the capture through the fork / join events)
+int capture_panel(lbm_ctx *c, int start) {
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+    hipError_t e = hipSuccess;
+    int cur = start;
+    for (int k = 0; k < c->graph_steps / 2 && e == hipSuccess; ++k) {
+        e = panel_launch(c, cur);
+        cur = 1 - cur;
+    }
+    hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+    if (e != hipSuccess) { if (g) (void)hipGraphDestroy(g); return hip_fail(e, "capture"); }
+    if (e2 != hipSuccess) return hip_fail(e2, "hipStreamEndCapture");
+    hipError_t e3 = hipGraphInstantiate(&c->pgraph[start], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e3 != hipSuccess) return hip_fail(e3, "hipGraphInstantiate");
+    return LBM_OK;
 }
 
 // capture graph_steps fused pull steps starting from buffer `start`
@@ -202,12 +237,22 @@ int step_impl(lbm_ctx *c, int variant, int nsteps) {
             if (rc) return rc;
             k = 1;
         }
+        const int gsteps = 2 * (c->graph_steps / 2);
+        if (variant == LBM_PANEL && gsteps > 0) {
+            while (nsteps - k >= gsteps) {
+                if (!c->pgraph[c->cur]) {
+                    int rc = capture_panel(c, c->cur);
+                    if (rc) return rc;
+                }
+                HIPCHK(hipGraphLaunch(c->pgraph[c->cur], c->stream));
+                if ((gsteps / 2) & 1) c->cur = 1 - c->cur;
+                c->steps += gsteps;
+                k += gsteps;
+            }
+        }
         for (; k + 2 <= nsteps; k += 2) {
             if (variant == LBM_PANEL) {
-                WaveGrid g = wave_grid(c->prob.lay, c->tile_x);
-                HIPCHK(launch_wpanel(c->storage, c->prob, g, c->buf[c->cur], c->buf[1 - c->cur],
-                                     c->stream, c->side, c->fork, c->join));
-                if (c->storage != LBM_STORE_F64) HIPCHK(launch_bump(c->d_step, 2, c->stream));
+                HIPCHK(panel_launch(c, c->cur));
             } else {
                 BlockGrid g = block_grid(c->prob.lay, c->tile_x,
                                          c->tile_y > 0 ? c->tile_y + 2 : 0);
@@ -470,6 +515,7 @@ int lbm_hip_device_memory(size_t *free_bytes, size_t *total_bytes) {
 
 int lbm_hip_set_tile(lbm_ctx *c, int tile_x, int tile_y) {
     if (!c || tile_x < 0 || tile_y < 0) return fail(LBM_ERR_ARG, "bad tile");
+    if (tile_x != c->tile_x || tile_y != c->tile_y) drop_graphs(c);
     c->tile_x = tile_x;
     c->tile_y = tile_y;
     return LBM_OK;

@@ -356,12 +356,19 @@ __global__ __launch_bounds__(64 * WPG) void k_wpanel_in(Problem p, WaveGrid g,
     static_for<DEPTH>([&]<int D>() { iter.template operator()<TAIL>(x + D, f[(2 + D) % DEPTH]); });
 }
 
-// Chunk width 16 unless given.  Short chunks keep the last round of waves
-// small at (cx + 2) / cx first-step work.  On 16384 x 4096 (FP64) chunks of
-// 16 give 59 GLUPS against 58 for 32, 57 for 64 and 55 for 128.
+// Chunk width, unless given: the widest even width in [4, 16] that still
+// yields about 16384 waves, so mid-size lattices get enough waves in flight
+// (2048 x 512 FP64: 38.7 GLUPS at 4 columns, 22.3 at 16) while large ones
+// keep the (cx + 2) / cx first-step overhead low (16384 x 4096: 59 GLUPS at
+// 16, 58 at 32, 55 at 128).
 WaveGrid wave_grid(const Layout &L, int cx) {
     WaveGrid g;
-    g.cx = cx >= 3 ? cx : 16;
+    if (cx < 3) {
+        const int64_t bands = (L.ly + WAVE_ROWS - 1) / WAVE_ROWS;
+        const int64_t c = bands * L.lx / 16384;
+        cx = (int)(c < 4 ? 4 : c > 16 ? 16 : c) & ~1;
+    }
+    g.cx = cx;
     // the interior kernel's buffer resources span one chunk of one plane and
     // must stay below the drop offset (2 GiB), at up to 8 bytes per element
     while (g.cx > 3 && (int64_t)g.cx * L.pitch * 8 >= (int64_t)DROP) g.cx /= 2;

@@ -13,7 +13,8 @@ server loop:
     work, a multiple of the captured graph length) on the solve's own HIP
     stream whenever its previous slice has finished, so solves of different
     sizes run concurrently on the GPU and a newly admitted request starts at
-    the next slice boundary instead of waiting for the queue to drain;
+    the next slice boundary instead of waiting for the queue to drain; large
+    lattices run the two-step panel sweep, small ones the fused sweep;
   * on completion takes the solve's diagnostics (and, if asked, its velocity
     field), frees its memory and records arrival -> completion latency.
 
@@ -103,10 +104,17 @@ class SolveServer:
     """Admission by device-memory footprint, device-resident lattices,
     time-sliced concurrent execution (one HIP stream per solve)."""
 
+    # lattices from this size up run the two-step panel sweep under
+    # variant="auto", smaller ones the fused sweep: on MI355X with graph
+    # replay the fused sweep is faster up to 1024 x 256 (22 vs 15 GLUPS FP64),
+    # about even at 2048 x 512 (FP64 32 vs 39, FP32 47 vs 42), and the panel
+    # from 4096 x 1024 (FP64 53 vs 37, FP32 76 vs 65)
+    PANEL_MIN_CELLS = 1_500_000
+
     def __init__(self, hbm_budget: Optional[int] = None, max_active: int = 8,
                  slice_ms: float = 20.0, graph_steps: int = 50,
                  est_glups: Dict[str, float] = None, keep_fields: bool = False,
-                 variant: str = "combine"):
+                 variant: str = "auto"):
         if hbm_budget is None:
             free, _ = device_memory()
             hbm_budget = int(0.9 * free)
@@ -118,6 +126,12 @@ class SolveServer:
         self.keep_fields = keep_fields
         self.variant = variant
         self.stats = {"admitted": 0, "peak_active": 0, "peak_bytes": 0, "slices": 0}
+
+    def _variant(self, req: Request) -> str:
+        if self.variant != "auto":
+            return self.variant
+        cells = req.channel.lx * req.channel.ly
+        return "panel" if cells >= self.PANEL_MIN_CELLS else "combine"
 
     def _slice(self, req: Request) -> int:
         cells = req.channel.lx * req.channel.ly
@@ -163,7 +177,7 @@ class SolveServer:
                 a.inflight = False
                 if a.remaining > 0:
                     n = min(a.slice_steps, a.remaining)
-                    a.lat.step(n, self.variant)
+                    a.lat.step(n, self._variant(a.req))
                     a.remaining -= n
                     a.inflight = True
                     self.stats["slices"] += 1
