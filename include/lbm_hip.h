@@ -48,13 +48,19 @@ enum {
  *   PANEL       two time steps per launch, each wave64 marching a band of
  *               rows with the first step held in registers
  *               (collide_tight_panel_iy, lb.c:1402-1779)
- * All five give bit-identical FP64 results, as the reference's serial ones do. */
+ *   PANEL4      four time steps per launch: the panel march with three levels
+ *               held in registers (no counterpart in the reference; the same
+ *               per-node operations in the same order).  Uses a third
+ *               population buffer; lattices too small for it run as PANEL.
+ * All of them give bit-identical FP64 results, as the reference's serial
+ * sweeps do. */
 typedef enum {
     LBM_ORIGIN = 0,
     LBM_COMBINE = 1,
     LBM_TIGHT = 2,
     LBM_TIGHT_BLOCK = 3,
-    LBM_PANEL = 4
+    LBM_PANEL = 4,
+    LBM_PANEL4 = 5
 } lbm_variant;
 
 /* Population storage.  F64 is the reference's arithmetic.  F32/F16 store the

@@ -38,6 +38,7 @@ struct lbm_ctx {
     size_t footprint;
     int graph_steps;               // fused steps per captured graph (0: eager)
     uint32_t *d_step;              // collision counter (keys FP32 stochastic rounding)
+    void *buf3;                    // third population buffer of the four-step sweep
     hipStream_t side;              // second stream for the panel's edge waves
     hipEvent_t fork, join;
     hipGraphExec_t graph[2];       // fused steps, by starting buffer
@@ -142,6 +143,14 @@ hipError_t panel_launch(lbm_ctx *c, int from) {
     return e;
 }
 
+// one four-step launch from buffer `from` into 1 - from (third buffer buf3)
+hipError_t panel4_launch(lbm_ctx *c, int from) {
+    hipError_t e = launch_wpanel4(c->storage, c->prob, c->tile_x, c->buf[from], c->buf3,
+                                  c->buf[1 - from], c->stream, c->side, c->fork, c->join);
+    if (e == hipSuccess && c->storage != LBM_STORE_F64) e = launch_bump(c->d_step, 4, c->stream);
+    return e;
+}
+
 // capture graph_steps / 2 panel launches starting from buffer `start` (the
 // perimeter stream joins the capture through the fork / join events)
 int capture_panel(lbm_ctx *c, int start) {
@@ -222,6 +231,31 @@ int step_impl(lbm_ctx *c, int variant, int nsteps) {
             c->steps += 1;
         }
         return LBM_OK;
+    }
+    case LBM_PANEL4: {
+        // four steps per launch while four remain, then the two-step panel
+        // and a fused single step for the rest
+        if (c->prob.lay.lx < 4)
+            return fail(LBM_ERR_UNSUPPORTED, "two-step variants need lx >= 4");
+        WaveGrid g = wave_grid(c->prob.lay, c->tile_x);
+        if (!wpanel4_possible(c->prob.lay, g.cx)) return step_impl(c, LBM_PANEL, nsteps);
+        if (!c->buf3) {
+            size_t bytes = 9 * (size_t)c->prob.lay.plane * elem_size(c->storage);
+            HIPCHK(hipMalloc(&c->buf3, bytes));
+            c->footprint += bytes;
+        }
+        int k = 0;
+        if (!c->collided) {
+            int rc = to_collided(c);
+            if (rc) return rc;
+            k = 1;
+        }
+        for (; k + 4 <= nsteps; k += 4) {
+            HIPCHK(panel4_launch(c, c->cur));
+            c->cur = 1 - c->cur;
+            c->steps += 4;
+        }
+        return k < nsteps ? step_impl(c, LBM_PANEL, nsteps - k) : LBM_OK;
     }
     case LBM_TIGHT_BLOCK:
     case LBM_PANEL: {
@@ -396,6 +430,7 @@ int lbm_hip_destroy(lbm_ctx *c) {
     if (c->d_partial) (void)hipFree(c->d_partial);
     if (c->d_out4) (void)hipFree(c->d_out4);
     if (c->d_step) (void)hipFree(c->d_step);
+    if (c->buf3) (void)hipFree(c->buf3);
     if (c->side) (void)hipStreamSynchronize(c->side);
     if (c->fork) (void)hipEventDestroy(c->fork);
     if (c->join) (void)hipEventDestroy(c->join);
@@ -606,6 +641,7 @@ double lbm_hip_bytes_per_node(const lbm_ctx *c, int variant) {
     case LBM_TIGHT: return 2 * pop;         // one read and one write per step
     case LBM_TIGHT_BLOCK:
     case LBM_PANEL: return pop;             // one read and one write per two steps
+    case LBM_PANEL4: return pop / 2;        // per four steps
     default: return 0.0;
     }
 }

@@ -121,8 +121,9 @@ __device__ __forceinline__ void edge_task(const WaveGrid &g, int e, int &band, i
     }
 }
 
+// dstep: index of this launch's first collision relative to the counter
 template <typename S>
-__global__ __launch_bounds__(64) void k_wpanel_edge(Problem p, WaveGrid g,
+__global__ __launch_bounds__(64) void k_wpanel_edge(Problem p, WaveGrid g, uint32_t dstep,
                                                     const S *__restrict__ P, S *__restrict__ Q) {
     using T = typename Calc<S>::T;
     const Layout &L = p.lay;
@@ -135,7 +136,7 @@ __global__ __launch_bounds__(64) void k_wpanel_edge(Problem p, WaveGrid g,
     const bool row_in = y >= 1 && y <= L.ly;
     const bool owner = lane >= 1 && lane <= WAVE_ROWS && y <= L.ly;
     const int s_row = (y == 1) ? LOWER : (y == L.ly) ? UPPER : -1;
-    const uint32_t m = sizeof(S) == 4 ? *p.step : 0u;   // index of the first collision
+    const uint32_t m = sizeof(S) == 4 ? *p.step + dstep : 0u;   // index of the first collision
 
     T cur[9], c1[9], c2[3];      // columns x, x-1 and (1, 5, 8 of) x-2
     S fn[9];                     // next column's pulled populations
@@ -247,7 +248,7 @@ __device__ __forceinline__ int xcd_block(int b, int nblk) {
 
 // Compile-time loop over K = 0 .. N-1.
 template <int N, typename F> __device__ __forceinline__ void static_for(F &&f) {
-    [&]<int... K>(std::integer_sequence<int, K...>) {
+    [&]<int... K>(std::integer_sequence<int, K...>) __attribute__((always_inline)) {
         (f.template operator()<K>(), ...);
     }(std::make_integer_sequence<int, N>{});
 }
@@ -257,103 +258,139 @@ template <int N, typename F> __device__ __forceinline__ void static_for(F &&f) {
 // memory together as one contiguous run per population.
 constexpr int WPG = 8;
 
-// DEPTH = columns whose loads are in flight (1 to 4); cx % DEPTH == 0.
-template <typename S, int DEPTH>
+// Interior waves, LEVELS time steps per launch (2, or 4 for the deeper
+// sweep).  A wave's 64 lanes are rows Y0-H .. Y0+63-H with H = LEVELS-1 halo
+// rows on each side: level l (1-based) is valid on lanes l-1 .. 64-l, so the
+// wave owns the 64-2H rows of the last level.  Iteration x computes level 1
+// of column x from the pulled populations and level l of column x-l+1 from
+// level l-1's window of three columns (two kept in registers per level, the
+// third just computed), then stores level LEVELS of column x-H.
+// DEPTH = columns whose loads are in flight (1 or 2); cx % DEPTH == 0.
+template <typename S, int DEPTH, int LEVELS>
 __global__ __launch_bounds__(64 * WPG) void k_wpanel_in(Problem p, WaveGrid g,
                                                         const S *__restrict__ P,
                                                         S *__restrict__ Q) {
     using T = typename Calc<S>::T;
+    constexpr int H = LEVELS - 1;
+    constexpr int ROWS = 64 - 2 * H;
     const Layout &L = p.lay;
     const int lane = threadIdx.x & 63;
     const int t = __builtin_amdgcn_readfirstlane(
         xcd_block(blockIdx.x, gridDim.x) * WPG + (threadIdx.x >> 6));
     if (t >= g.nin) return;                              // whole waves only
-    const int nbi = g.b_hi - g.b_lo;
-    const WaveSpan w = wave_span(L, g, g.b_lo + t % nbi, g.c_lo + t / nbi);
-    const int y = w.Y0 - 1 + lane;
+    const int ck = g.c_lo + t / g.nbl;
+    const int Y0 = g.r0 + (t % g.nbl) * ROWS;
+    const int X1 = L.lx + 1 - (g.ncx - 1 - ck) * g.cx;  // interior chunks are full
+    const int X0 = X1 - g.cx;
+    const int y = Y0 - H + lane;
+    // a partial last band: lanes past the interior's halo read row ly (in
+    // bounds; their values reach no owned row)
+    const int yl = y < L.ly ? y : L.ly;
     const uint32_t m = sizeof(S) == 4 ? *p.step : 0u;   // index of the first collision
     const uint32_t colb = (uint32_t)L.pitch * (uint32_t)sizeof(S);
-    const uint32_t lane_off =
-        (lane >= 1 && lane <= WAVE_ROWS) ? (uint32_t)(y + L.yoff) * (uint32_t)sizeof(S) : DROP;
+    const uint32_t lane_off = (lane >= H && lane < H + ROWS && y < g.r1)
+                                  ? (uint32_t)(y + L.yoff) * (uint32_t)sizeof(S) : DROP;
     // per population: columns [X0, X1) of Q
     __amdgpu_buffer_rsrc_t rs[9];
     for_pops([&]<int I>() {
         rs[I] = __builtin_amdgcn_make_buffer_rsrc(
-            (void *)(Q + I * L.plane + (int64_t)w.X0 * L.pitch), 0,
-            (int)((uint32_t)(w.X1 - w.X0) * colb), RSRC_WORD3);
+            (void *)(Q + I * L.plane + (int64_t)X0 * L.pitch), 0,
+            (int)((uint32_t)(X1 - X0) * colb), RSRC_WORD3);
     });
 
     S f[DEPTH][9];
-    T cur[9], c1[9], c2[3];
+    // per level 1..LEVELS-1: columns x-l+1 (cur), x-l (c1) and populations
+    // 1, 5, 8 of x-l-1 (c2); index 0 unused
+    T cur[LEVELS][9], c1[LEVELS][9], c2[LEVELS][3];
     for (int d = 0; d < DEPTH; ++d)
         for (int i = 0; i < 9; ++i) f[d][i] = 0;
-    for (int i = 0; i < 9; ++i) { cur[i] = 0; c1[i] = 0; }
-    c2[0] = c2[1] = c2[2] = 0;
+    for (int l = 0; l < LEVELS; ++l) {
+        for (int i = 0; i < 9; ++i) { cur[l][i] = 0; c1[l][i] = 0; }
+        c2[l][0] = c2[l][1] = c2[l][2] = 0;
+    }
 
     // per population, a wave-uniform base (SGPRs) with pb[I][v(x)] the value
     // pulled into (x, y), v(x) = (x - X0 + 3) * pitch + y + yoff: one 32-bit
     // lane offset per column serves all nine loads
     const S *pb[9];
     for_pops([&]<int I>() {
-        pb[I] = P + I * L.plane + (int64_t)(w.X0 - 3 - cx(I)) * L.pitch - cy(I);
+        pb[I] = P + I * L.plane + (int64_t)(X0 - 3 - cx(I)) * L.pitch - cy(I);
     });
-    const uint32_t vy = (uint32_t)(y + L.yoff) * (uint32_t)sizeof(S);
+    const uint32_t vy = (uint32_t)(yl + L.yoff) * (uint32_t)sizeof(S);
     auto pull = [&](int x, S (&dst)[9]) {
         // byte offset kept 32-bit so the loads take the SGPR-base form
-        const uint32_t v = (uint32_t)(x - w.X0 + 3) * colb + vy;
+        const uint32_t v = (uint32_t)(x - X0 + 3) * colb + vy;
         for_pops([&]<int I>() {
             dst[I] = *reinterpret_cast<const S *>(reinterpret_cast<const char *>(pb[I]) + v);
         });
     };
-    // Iteration x: step m of column x, step m+1 of column x-1, stores of
-    // column x-1; LOAD pulls column x + DEPTH into the buffer just consumed.
-    // Kinds: FILL iterations only put loads in flight, HEAD iterations compute
-    // the first step of columns X0-1 and X0, FULL iterations do both steps,
-    // TAIL iterations are the last DEPTH, with nothing left to load.  Every
-    // kind issues the same stores (FILL and HEAD ones dropped), so the memory
-    // traffic of the loop and of the code before it looks alike to the
-    // compiler's wait-count analysis.
+    // One iteration: level 1 of column x, then levels 2 .. TOP (TOP < LEVELS
+    // in the head iterations, where the higher levels' columns lie outside
+    // what the chunk needs), the store of level LEVELS (dropped unless
+    // TOP == LEVELS), and the window shifts.  FILL iterations only put loads
+    // in flight; TAIL iterations are the last DEPTH, with nothing left to
+    // load.  Every kind issues the same stores, so the memory traffic of the
+    // loop and of the code around it looks alike to the compiler's wait-count
+    // analysis.
     enum { FILL, HEAD, FULL, TAIL };
-    auto iter = [&]<int KIND>(int x, S (&buf)[9]) {
+    auto iter = [&]<int KIND, int TOP>(int x, S (&buf)[9]) __attribute__((always_inline)) {
         if constexpr (KIND == FILL) {
             pull(x + DEPTH, buf);
             for_pops([&]<int I>() { buffer_store<S>(S(0), rs[I], DROP); });
         } else {
-            for_pops([&]<int I>() { cur[I] = as_calc<S>(buf[I]); });
+            for_pops([&]<int I>() { cur[1][I] = as_calc<S>(buf[I]); });
             if constexpr (KIND != TAIL) pull(x + DEPTH, buf);
-            collide_any<S>(p, -1, x, y, cur, 0., m);
-            for_pops([&]<int I>() { cur[I] = stored<S>(cur[I]); });
-            if constexpr (KIND == HEAD) {
+            collide_any<S>(p, -1, x, y, cur[1], 0., m);
+            for_pops([&]<int I>() { cur[1][I] = stored<S>(cur[1][I]); });
+            static_for<TOP - 1>([&]<int K>() __attribute__((always_inline)) {
+                constexpr int l = K + 2;                 // level computed, column x-l+1
+                constexpr int b = l - 1;                 // the level it gathers from
+                const T up2 = lane_up(c1[b][2]), up5 = lane_up(c2[b][1]), up6 = lane_up(cur[b][6]);
+                const T dn4 = lane_down(c1[b][4]), dn7 = lane_down(cur[b][7]), dn8 = lane_down(c2[b][2]);
+                T h[9] = {c1[b][0], c2[b][0], up2, cur[b][3], dn4, up5, up6, dn7, dn8};
+                collide_any<S>(p, -1, x - l + 1, y, h, 0., m + l - 1);
+                if constexpr (l < LEVELS) {
+                    for_pops([&]<int I>() { cur[l][I] = stored<S>(h[I]); });
+                } else {
+                    const uint32_t off = lane_off + (uint32_t)(x - H - X0) * colb;
+                    for_pops([&]<int I>() { buffer_store<S>(to_storage<S>(h[I]), rs[I], off); });
+                }
+            });
+            if constexpr (TOP < LEVELS)
                 for_pops([&]<int I>() { buffer_store<S>(S(0), rs[I], DROP); });
-            } else {
-                const T up2 = lane_up(c1[2]), up5 = lane_up(c2[1]), up6 = lane_up(cur[6]);
-                const T dn4 = lane_down(c1[4]), dn7 = lane_down(cur[7]), dn8 = lane_down(c2[2]);
-                T h[9] = {c1[0], c2[0], up2, cur[3], dn4, up5, up6, dn7, dn8};
-                collide_any<S>(p, -1, x - 1, y, h, 0., m + 1);
-                const uint32_t off = lane_off + (uint32_t)(x - 1 - w.X0) * colb;
-                for_pops([&]<int I>() { buffer_store<S>(to_storage<S>(h[I]), rs[I], off); });
-            }
-            c2[0] = c1[1];
-            c2[1] = c1[5];
-            c2[2] = c1[8];
-            for_pops([&]<int I>() { c1[I] = cur[I]; });
+            // shift the windows of the levels computed in this iteration
+            static_for<(TOP < LEVELS ? TOP : LEVELS - 1)>([&]<int K>() __attribute__((always_inline)) {
+                constexpr int l = K + 1;
+                c2[l][0] = c1[l][1];
+                c2[l][1] = c1[l][5];
+                c2[l][2] = c1[l][8];
+                for_pops([&]<int I>() { c1[l][I] = cur[l][I]; });
+            });
         }
         __builtin_amdgcn_s_barrier();
     };
 
-    // x runs X0-1-DEPTH .. X1: DEPTH fill iterations, HEAD at X0-1 and X0,
-    // FULL up to X1-DEPTH, then DEPTH tail iterations; the j-th iteration
-    // (from 0) consumes and reloads buffer j % DEPTH
-    int x = w.X0 - 1 - DEPTH;
+    // x runs X0-H-DEPTH .. X1+H-1: DEPTH fill iterations, 2H head iterations
+    // (head i computes levels up to i/2 + 1), FULL up to X1+H-1-DEPTH, then
+    // DEPTH tail iterations; the j-th iteration (from 0) consumes and reloads
+    // buffer j % DEPTH
+    int x = X0 - H - DEPTH;
     if constexpr (sizeof(S) == 4) fp32_round_toward_zero();
-    static_for<DEPTH>([&]<int D>() { iter.template operator()<FILL>(x + D, f[D]); });
+    static_for<DEPTH>([&]<int D>() __attribute__((always_inline)) {
+        iter.template operator()<FILL, 1>(x + D, f[D]);
+    });
     x += DEPTH;
-    iter.template operator()<HEAD>(x, f[0]);
-    iter.template operator()<HEAD>(x + 1, f[1 % DEPTH]);
-    x += 2;
-    for (int k = 0; k < (w.X1 - w.X0) / DEPTH - 1; ++k, x += DEPTH)
-        static_for<DEPTH>([&]<int D>() { iter.template operator()<FULL>(x + D, f[(2 + D) % DEPTH]); });
-    static_for<DEPTH>([&]<int D>() { iter.template operator()<TAIL>(x + D, f[(2 + D) % DEPTH]); });
+    static_for<2 * H>([&]<int I>() __attribute__((always_inline)) {
+        iter.template operator()<HEAD, I / 2 + 1>(x + I, f[I % DEPTH]);
+    });
+    x += 2 * H;
+    for (int k = 0; k < (X1 - X0) / DEPTH - 1; ++k, x += DEPTH)
+        static_for<DEPTH>([&]<int D>() __attribute__((always_inline)) {
+            iter.template operator()<FULL, LEVELS>(x + D, f[(2 * H + D) % DEPTH]);
+        });
+    static_for<DEPTH>([&]<int D>() __attribute__((always_inline)) {
+        iter.template operator()<TAIL, LEVELS>(x + D, f[(2 * H + D) % DEPTH]);
+    });
 }
 
 // Chunk width, unless given: the widest even width in [4, 16] that still
@@ -390,7 +427,41 @@ WaveGrid wave_grid(const Layout &L, int cx) {
     g.b_lo = b_lo; g.b_hi = b_hi; g.c_lo = c_lo; g.c_hi = c_hi;
     g.nin = (b_hi - b_lo) * (c_hi - c_lo);
     g.nedge = g.ntasks - g.nin;
+    g.r0 = 1 + b_lo * WAVE_ROWS;
+    g.r1 = 1 + b_hi * WAVE_ROWS;
+    g.nbl = b_hi - b_lo;
     return g;
+}
+
+// The grid of the four-step sweep: the interior rectangle of the two-step
+// grid, shrunk until its three-node halo stays off the perimeter (rows and
+// columns 2 .. ly-1 / lx-1), tiled by 58-row bands (the last one partial).
+static WaveGrid wave_grid4(const Layout &L, int cx) {
+    WaveGrid g = wave_grid(L, cx);
+    constexpr int ROWS4 = 64 - 2 * 3;
+    auto chunk_x0 = [&](int ck) { return L.lx + 1 - (g.ncx - 1 - ck) * g.cx - g.cx; };
+    while (g.b_hi > g.b_lo && 1 + g.b_hi * WAVE_ROWS + 2 > L.ly - 1) --g.b_hi;
+    while (g.c_lo < g.c_hi && chunk_x0(g.c_lo) - 3 < 2) ++g.c_lo;
+    while (g.c_hi > g.c_lo && chunk_x0(g.c_hi - 1) + g.cx + 2 > L.lx - 1) --g.c_hi;
+    if (g.b_lo >= g.b_hi || g.c_lo >= g.c_hi) {
+        g.b_lo = g.b_hi = g.c_lo = g.c_hi = 0;
+        g.nin = 0;
+        g.nedge = g.ntasks;
+        g.nbl = 0;
+        return g;
+    }
+    g.r0 = 1 + g.b_lo * WAVE_ROWS;
+    g.r1 = 1 + g.b_hi * WAVE_ROWS;
+    g.nbl = (g.r1 - g.r0 + ROWS4 - 1) / ROWS4;
+    g.nin = g.nbl * (g.c_hi - g.c_lo);
+    g.nedge = g.ntasks - (g.b_hi - g.b_lo) * (g.c_hi - g.c_lo);
+    return g;
+}
+
+bool wpanel4_possible(const Layout &L, int cx) {
+    WaveGrid g = wave_grid4(L, cx);
+    // the widened frame of the first perimeter launch must leave an interior
+    return g.b_hi - g.b_lo > 2 && g.c_hi - g.c_lo > 2;
 }
 
 // Perimeter waves on `side`, interior waves on `st`, joined back into `st`:
@@ -407,12 +478,43 @@ hipError_t launch_wpanel(int storage, const Problem &p, const WaveGrid &g, const
     if (e != hipSuccess) return e;
     auto go = [&](auto tag) {
         using S = decltype(tag);
-        if (g.nedge) k_wpanel_edge<S><<<g.nedge, 64, 0, side>>>(p, g, (const S *)P, (S *)Q);
+        if (g.nedge) k_wpanel_edge<S><<<g.nedge, 64, 0, side>>>(p, g, 0u, (const S *)P, (S *)Q);
         if (!g.nin) return;
         const S *src = (const S *)P;
         S *dst = (S *)Q;
-        if (depth == 2) k_wpanel_in<S, 2><<<blocks, 64 * WPG, 0, st>>>(p, g, src, dst);
-        else k_wpanel_in<S, 1><<<blocks, 64 * WPG, 0, st>>>(p, g, src, dst);
+        if (depth == 2) k_wpanel_in<S, 2, 2><<<blocks, 64 * WPG, 0, st>>>(p, g, src, dst);
+        else k_wpanel_in<S, 1, 2><<<blocks, 64 * WPG, 0, st>>>(p, g, src, dst);
+    };
+    if (storage == 0) go(double{});
+    else if (storage == 1) go(float{});
+    else go(_Float16{});
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventRecord(join, side);
+    if (e == hipSuccess) e = hipStreamWaitEvent(st, join, 0);
+    return e;
+}
+
+hipError_t launch_wpanel4(int storage, const Problem &p, int cx, const void *P, void *S, void *Q,
+                          hipStream_t st, hipStream_t side, hipEvent_t fork, hipEvent_t join) {
+    const Layout &L = p.lay;
+    WaveGrid g = wave_grid4(L, cx);
+    if (g.b_hi - g.b_lo <= 2 || g.c_hi - g.c_lo <= 2) return hipErrorInvalidValue;
+    // frame of the first perimeter launch: one band / chunk wider, so that its
+    // output in S covers every value the second launch pulls
+    WaveGrid ga = g;
+    ga.b_lo += 1; ga.b_hi -= 1; ga.c_lo += 1; ga.c_hi -= 1;
+    ga.nedge = ga.ntasks - (ga.b_hi - ga.b_lo) * (ga.c_hi - ga.c_lo);
+    const int depth = g.cx % 2 == 0 ? 2 : 1;
+    const int blocks = (g.nin + WPG - 1) / WPG;
+    hipError_t e = hipEventRecord(fork, st);
+    if (e == hipSuccess) e = hipStreamWaitEvent(side, fork, 0);
+    if (e != hipSuccess) return e;
+    auto go = [&](auto tag) {
+        using T = decltype(tag);
+        k_wpanel_edge<T><<<ga.nedge, 64, 0, side>>>(p, ga, 0u, (const T *)P, (T *)S);
+        k_wpanel_edge<T><<<g.nedge, 64, 0, side>>>(p, g, 2u, (const T *)S, (T *)Q);
+        if (depth == 2) k_wpanel_in<T, 2, 4><<<blocks, 64 * WPG, 0, st>>>(p, g, (const T *)P, (T *)Q);
+        else k_wpanel_in<T, 1, 4><<<blocks, 64 * WPG, 0, st>>>(p, g, (const T *)P, (T *)Q);
     };
     if (storage == 0) go(double{});
     else if (storage == 1) go(float{});

@@ -58,6 +58,8 @@ struct WaveGrid {
     int nbands, ncx, ntasks;
     int b_lo, b_hi, c_lo, c_hi;    // (band, chunk) rectangle of the interior waves
     int nin, nedge;                // interior waves, perimeter waves
+    int r0, r1, nbl;               // interior rows [r0, r1) in nbl bands of the
+                                   // interior kernel's height
 };
 
 // Rows [Y0-1, Y0+62] (lanes) and columns [X0, X1) owned by wave (band, ck);
@@ -74,5 +76,12 @@ __host__ __device__ inline WaveSpan wave_span(const Layout &L, const WaveGrid &g
 WaveGrid wave_grid(const Layout &L, int cx);
 hipError_t launch_wpanel(int storage, const Problem &p, const WaveGrid &g, const void *P, void *Q,
                          hipStream_t st, hipStream_t side, hipEvent_t fork, hipEvent_t join);
+// Four steps per launch: the interior waves march four levels in registers
+// (P -> Q); the perimeter frame runs two two-step launches through the third
+// buffer S (P -> S over the frame widened by one band / chunk, then S -> Q).
+// Returns hipErrorInvalidValue when the lattice has no interior for it.
+bool wpanel4_possible(const Layout &L, int cx);
+hipError_t launch_wpanel4(int storage, const Problem &p, int cx, const void *P, void *S, void *Q,
+                          hipStream_t st, hipStream_t side, hipEvent_t fork, hipEvent_t join);
 
 }  // namespace lbm

@@ -157,6 +157,54 @@ def test_panel_interior_waves(orc):
         assert np.array_equal(po[2:260], pref[2:260])
 
 
+@pytest.mark.parametrize("geom", ["A", "B"])
+def test_panel4_against_oracle(orc, geom):
+    """Four steps per launch (three levels in registers, the perimeter frame
+    through the third buffer) against the oracle, bit for bit, for step
+    counts that leave every remainder (a two-step launch, a single fused
+    step) and for chunk widths with one and two columns in flight."""
+    ch = OChannel.geometry(600, 400, geom)
+    for steps in (4, 5, 6, 7, 13):
+        _, lref, tref, pref = orc.run(ch, steps)
+        for cxk in (0, 5, 8, 16):
+            with L.Lattice(dev_channel(ch)) as lat:
+                lat.set_tile(cxk, 0)
+                lat.init_equilibrium()
+                lat.step(steps, "panel4")
+                lo, to, po = lat.download()
+            what = f"panel4 {geom} steps {steps} chunk {cxk}"
+            assert_same(lo, lref, what)
+            assert_same(to, tref, what + " tmp")
+            assert np.array_equal(po[2:400], pref[2:400]), what
+
+
+def test_panel4_long_run_and_switches(orc):
+    """200 steps of panel4 mixed with the other sweeps on a lattice whose
+    last band of interior rows is partial (ly - 1 not a multiple of the band
+    height), against the oracle."""
+    ch = OChannel.geometry(700, 333, "B")
+    _, lref, tref, _ = orc.run(ch, 200)
+    with L.Lattice(dev_channel(ch)) as lat:
+        lat.init_equilibrium()
+        for n, v in ((37, "panel4"), (3, "combine"), (80, "panel4"), (2, "panel"), (78, "panel4")):
+            lat.step(n, v)
+        lo, to, _ = lat.download()
+    assert_same(lo, lref, "panel4 mixed")
+    assert_same(to, tref, "panel4 mixed tmp")
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_panel4_reduced_equals_single_steps(storage):
+    """In the reduced formats four steps per launch give the bits of four
+    single steps (FP32: the same stochastic rounding per collision index)."""
+    ch = L.Channel(600, 400, 120, 200, 40)
+    with L.Lattice(ch, storage=storage) as a, L.Lattice(ch, storage=storage) as b:
+        a.init_equilibrium(); b.init_equilibrium()
+        a.step(41, "combine"); b.step(41, "panel4")
+        la, ta, pa = a.download(); lb, tb, pb = b.download()
+    assert np.array_equal(la, lb) and np.array_equal(ta, tb) and np.array_equal(pa, pb)
+
+
 @pytest.mark.parametrize("variant,tiles", [
     ("panel", [(0, 0), (3, 5), (7, 254), (130, 17), (4, 1)]),
     ("tight_block", [(0, 0), (3, 30), (5, 62), (20, 126), (50, 62)]),
