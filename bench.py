@@ -5,8 +5,8 @@ Re 100, u_max 0.02, regularized walls, Poiseuille inlet, zero-gradient
 pressure outlet -- with FP64 populations, bit-identical to the reference C
 code.  W warm-up steps, then K timed steps; one JSON line on stdout.
 
-`value` is the GLUPS of the headline sweep variant (default panel: two time
-steps per launch, the first held in wave64 registers).  The other variants are timed on the same
+`value` is the GLUPS of the headline sweep variant (default panel4: four time
+steps per launch, three of them held in wave64 registers).  The other variants are timed on the same
 lattice and reported under "variants".  The populations are initialised on
 the device, so everything is resident in HBM when the timed region starts.
 
@@ -40,12 +40,13 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "GLUPS & % of HBM roofline (FP64 vs FP32-store); serving p50/p99 latency"
 KERNEL_OF = {"origin": "k_collide<false>", "combine": "k_collide<true>",
              "tight": "k_collide<true>", "tight_block": "k_block<64>",
-             "panel": "k_wpanel_in<double, 2>"}
-LOWP_KERNEL_OF = {("f32", "combine"): "k_collide_lp<float, true>",
-                  ("f16", "combine"): "k_collide_lp<_Float16, true>",
-                  ("f32", "panel"): "k_wpanel_in<float, 2>",
-                  ("f16", "panel"): "k_wpanel_in<_Float16, 2>"}
-UPDATES_PER_LAUNCH = {"origin": 0.5, "combine": 1, "tight": 1, "tight_block": 2, "panel": 2}
+             "panel": "k_wpanel_in<double, 2, 2>", "panel4": "k_wpanel_in<double, 2, 4>"}
+LOWP_KERNEL_OF = {(s, v): k.replace("double", {"f32": "float", "f16": "_Float16"}[s])
+                  for s in ("f32", "f16") for v, k in
+                  (("combine", "k_collide_lp<double, true>"), ("panel", "k_wpanel_in<double, 2, 2>"),
+                   ("panel4", "k_wpanel_in<double, 2, 4>"))}
+UPDATES_PER_LAUNCH = {"origin": 0.5, "combine": 1, "tight": 1, "tight_block": 2, "panel": 2,
+                      "panel4": 4}
 PROFILE = os.path.join(ROOT, "profiles", "kernels_16384x4096.md")
 
 
@@ -56,8 +57,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--lx", type=int, default=16384)
     ap.add_argument("--ly", type=int, default=4096)
-    ap.add_argument("--variant", default="panel")
-    ap.add_argument("--variants", default="origin,combine,tight_block",
+    ap.add_argument("--variant", default="panel4")
+    ap.add_argument("--variants", default="origin,combine,tight_block,panel",
                     help="variants timed besides the headline one (comma list, or '')")
     ap.add_argument("--other-steps", type=int, default=1000)
     ap.add_argument("--storages", default="f32,f16",
@@ -91,8 +92,9 @@ def time_variant(lbm, ch, variant, steps, warmup, storage="f64"):
                 launch_ms=ms / launches)
 
 
-# kernels of one launch of a variant whose HBM traffic adds up to the launch's
-TRAFFIC_KERNELS = {"panel": ("k_wpanel_in<double, 2>", "k_wpanel_edge<double>")}
+# entry of the profile's traffic line holding one launch of a variant (the
+# panel sweeps' interior and perimeter dispatches added up)
+TRAFFIC_KEY = {"panel": "panel_launch<double>", "panel4": "panel4_launch<double>"}
 
 
 def measured_traffic(variant):
@@ -106,10 +108,7 @@ def measured_traffic(variant):
         return None
     # names hold ", " themselves (template arguments): split at "=<digits>"
     table = {k.strip(): float(v) for k, v in re.findall(r"([^=]+)=(\d+)(?:,\s*)?", m.group(1))}
-    names = TRAFFIC_KERNELS.get(variant, (KERNEL_OF[variant],))
-    if not all(n in table for n in names):
-        return None
-    return sum(table[n] for n in names)
+    return table.get(TRAFFIC_KEY.get(variant, KERNEL_OF[variant]))
 
 
 def batching(args):
@@ -232,7 +231,7 @@ def main():
     for sname in [v for v in args.storages.split(",") if v]:
         bpu = {"f32": 72.0, "f16": 36.0}[sname]
         storage[sname] = {"bytes_per_update": bpu}
-        for v in ("combine", "panel"):
+        for v in ("combine", "panel", "panel4"):
             r = time_variant(lbm, ch, v, args.other_steps, min(args.warmup, 20), sname)
             ach = bpu * cells * UPDATES_PER_LAUNCH[v] / (r["launch_ms"] * 1e-3) / 1e9
             storage[sname][v] = {"glups": round(r["glups"], 4),

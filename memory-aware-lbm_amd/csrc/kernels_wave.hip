@@ -398,12 +398,12 @@ __global__ __launch_bounds__(64 * WPG) void k_wpanel_in(Problem p, WaveGrid g,
 // (2048 x 512 FP64: 38.7 GLUPS at 4 columns, 22.3 at 16) while large ones
 // keep the (cx + 2) / cx first-step overhead low (16384 x 4096: 59 GLUPS at
 // 16, 58 at 32, 55 at 128).
-WaveGrid wave_grid(const Layout &L, int cx) {
+static WaveGrid wave_grid_max(const Layout &L, int cx, int cmax) {
     WaveGrid g;
     if (cx < 3) {
         const int64_t bands = (L.ly + WAVE_ROWS - 1) / WAVE_ROWS;
         const int64_t c = bands * L.lx / 16384;
-        cx = (int)(c < 4 ? 4 : c > 16 ? 16 : c) & ~1;
+        cx = (int)(c < 4 ? 4 : c > cmax ? cmax : c) & ~1;
     }
     g.cx = cx;
     // the interior kernel's buffer resources span one chunk of one plane and
@@ -433,11 +433,15 @@ WaveGrid wave_grid(const Layout &L, int cx) {
     return g;
 }
 
+WaveGrid wave_grid(const Layout &L, int cx) { return wave_grid_max(L, cx, 16); }
+
 // The grid of the four-step sweep: the interior rectangle of the two-step
 // grid, shrunk until its three-node halo stays off the perimeter (rows and
 // columns 2 .. ly-1 / lx-1), tiled by 58-row bands (the last one partial).
+// The four-step march does twice the arithmetic per column, so wider chunks
+// pay (16384 x 4096 FP64: 100 GLUPS at 20-32 columns, 91 at 8).
 static WaveGrid wave_grid4(const Layout &L, int cx) {
-    WaveGrid g = wave_grid(L, cx);
+    WaveGrid g = wave_grid_max(L, cx, 24);
     constexpr int ROWS4 = 64 - 2 * 3;
     auto chunk_x0 = [&](int ck) { return L.lx + 1 - (g.ncx - 1 - ck) * g.cx - g.cx; };
     while (g.b_hi > g.b_lo && 1 + g.b_hi * WAVE_ROWS + 2 > L.ly - 1) --g.b_hi;
