@@ -398,11 +398,11 @@ __global__ __launch_bounds__(64 * WPG) void k_wpanel_in(Problem p, WaveGrid g,
 // (2048 x 512 FP64: 38.7 GLUPS at 4 columns, 22.3 at 16) while large ones
 // keep the (cx + 2) / cx first-step overhead low (16384 x 4096: 59 GLUPS at
 // 16, 58 at 32, 55 at 128).
-static WaveGrid wave_grid_max(const Layout &L, int cx, int cmax) {
+static WaveGrid wave_grid_max(const Layout &L, int cx, int cmax, int64_t waves) {
     WaveGrid g;
     if (cx < 3) {
         const int64_t bands = (L.ly + WAVE_ROWS - 1) / WAVE_ROWS;
-        const int64_t c = bands * L.lx / 16384;
+        const int64_t c = bands * L.lx / waves;
         cx = (int)(c < 4 ? 4 : c > cmax ? cmax : c) & ~1;
     }
     g.cx = cx;
@@ -433,15 +433,18 @@ static WaveGrid wave_grid_max(const Layout &L, int cx, int cmax) {
     return g;
 }
 
-WaveGrid wave_grid(const Layout &L, int cx) { return wave_grid_max(L, cx, 16); }
+WaveGrid wave_grid(const Layout &L, int cx) { return wave_grid_max(L, cx, 16, 16384); }
 
 // The grid of the four-step sweep: the interior rectangle of the two-step
 // grid, shrunk until its three-node halo stays off the perimeter (rows and
 // columns 2 .. ly-1 / lx-1), tiled by 58-row bands (the last one partial).
-// The four-step march does twice the arithmetic per column, so wider chunks
-// pay (16384 x 4096 FP64: 100 GLUPS at 20-32 columns, 91 at 8).
+// The four-step march recomputes six halo columns per chunk and does twice
+// the arithmetic per column, so it wants wider chunks, up to 24, and fewer
+// waves (about 6144) before narrowing them: FP64 GLUPS at 4 / 8 / 12 / 24
+// columns: 4096 x 1024 56 / 65 / 66 / 51, 8192 x 2048 66 / 80 / 85 / 86,
+// 16384 x 4096 - / 91 / 97 / 100.
 static WaveGrid wave_grid4(const Layout &L, int cx) {
-    WaveGrid g = wave_grid_max(L, cx, 24);
+    WaveGrid g = wave_grid_max(L, cx, 24, 6144);
     constexpr int ROWS4 = 64 - 2 * 3;
     auto chunk_x0 = [&](int ck) { return L.lx + 1 - (g.ncx - 1 - ck) * g.cx - g.cx; };
     while (g.b_hi > g.b_lo && 1 + g.b_hi * WAVE_ROWS + 2 > L.ly - 1) --g.b_hi;

@@ -14,7 +14,7 @@ server loop:
     stream whenever its previous slice has finished, so solves of different
     sizes run concurrently on the GPU and a newly admitted request starts at
     the next slice boundary instead of waiting for the queue to drain; large
-    lattices run the two-step panel sweep, small ones the fused sweep;
+    lattices run the panel sweeps, small ones the fused sweep;
   * on completion takes the solve's diagnostics (and, if asked, its velocity
     field), frees its memory and records arrival -> completion latency.
 
@@ -104,12 +104,13 @@ class SolveServer:
     """Admission by device-memory footprint, device-resident lattices,
     time-sliced concurrent execution (one HIP stream per solve)."""
 
-    # lattices from this size up run the two-step panel sweep under
-    # variant="auto", smaller ones the fused sweep: on MI355X with graph
-    # replay the fused sweep is faster up to 1024 x 256 (22 vs 15 GLUPS FP64),
-    # about even at 2048 x 512 (FP64 32 vs 39, FP32 47 vs 42), and the panel
-    # from 4096 x 1024 (FP64 53 vs 37, FP32 76 vs 65)
+    # variant="auto" picks by lattice size: the fused sweep below
+    # PANEL_MIN_CELLS, the two-step panel up to PANEL4_MIN_CELLS, the
+    # four-step panel above.  Measured on MI355X (FP64 GLUPS, fused / panel /
+    # panel4): 1024 x 256 22.8 / 15.3 / 15.5, 2048 x 512 31.6 / 38.5 / 36,
+    # 4096 x 1024 36.8 / 53.4 / 66, 8192 x 2048 36.6 / 55.2 / 86.
     PANEL_MIN_CELLS = 1_500_000
+    PANEL4_MIN_CELLS = 3_000_000
 
     def __init__(self, hbm_budget: Optional[int] = None, max_active: int = 8,
                  slice_ms: float = 20.0, graph_steps: int = 50,
@@ -131,6 +132,8 @@ class SolveServer:
         if self.variant != "auto":
             return self.variant
         cells = req.channel.lx * req.channel.ly
+        if cells >= self.PANEL4_MIN_CELLS:
+            return "panel4"
         return "panel" if cells >= self.PANEL_MIN_CELLS else "combine"
 
     def _slice(self, req: Request) -> int:
