@@ -1,33 +1,37 @@
-// kernels_wave.hip -- two time steps per launch with the first step's results
-// held in registers of one wave64 (the panel sweep, collide_tight_panel_iy,
-// lb.c:1402-1538, mapped onto CDNA wavefronts).
+// kernels_wave.hip -- several time steps per launch with the intermediate
+// steps held in registers of one wave64: the panel sweep
+// (collide_tight_panel_iy, lb.c:1402-1538, two steps) mapped onto CDNA
+// wavefronts, and the same march four steps deep (no counterpart in the
+// reference).
 //
-// A wave owns a band of 62 rows and a chunk of columns, and marches x.  Its
-// 64 lanes cover the band plus one halo row on each side (lane l <-> row
-// Y0-1+l).  Iteration x computes the first step (m) of column x for all 64
-// rows into registers, then the second step (m+1) of column x-1 for the 62
-// owned rows.  The second step needs the first step's populations of
-// columns x-2, x-1 and x at rows y-1, y, y+1: a lane keeps its own rows'
-// values of the last two columns in registers and takes its neighbours'
-// with one lane shift up and one down (6 values per column).  Nothing goes
-// through LDS and no barrier is needed; waves are independent.
+// A wave owns a band of rows and a chunk of columns, and marches x.  For two
+// steps its 64 lanes are the band's 62 rows plus one halo row on each side
+// (lane l <-> row Y0-1+l).  Iteration x computes the first step (m) of
+// column x for all 64 rows into registers, then the second step (m+1) of
+// column x-1 for the 62 owned rows.  The second step needs the first step's
+// populations of columns x-2, x-1 and x at rows y-1, y, y+1: a lane keeps its
+// own rows' values of the last two columns in registers (column x-1: all 9
+// populations; x-2: populations 1, 5, 8, the cx = +1 sources) and takes its
+// neighbours' with one lane shift up and one down (6 values per column).
+// Nothing goes through LDS.  Four steps stack three such levels (58 owned
+// rows, three halo rows each side).
 //
-// Per column and lane: columns x-1 (all 9 populations: they are the second
-// step's cx = 0 sources next iteration and its unknown populations at the
-// walls) and x-2 (populations 1, 5, 8: the cx = +1 sources) stay resident.
+// Waves whose rows or columns touch the channel perimeter run a separate,
+// general kernel (k_wpanel_edge: wall operators, outlet extrapolation,
+// unknown populations); the interior kernel (k_wpanel_in) runs BGK /
+// bounce-back only, branch-free.  The second step's outlet density is
+// extrapolated from the post-stream densities of columns lx-2 and lx-1
+// gathered by the same lane just before their second collision
+// (lb.c:968-977); chunks are right-aligned so the chunk holding lx also holds
+// lx-2 and lx-1.  Unknown populations and the ghost ring follow the
+// collided-state convention of kernels_f64.hip: the first step pulls them
+// from P's ghost ring; the second step's are P(x) itself; the output Q's
+// ghost ring gets the first step's values.
 //
-// The second step's outlet density is extrapolated from the post-stream
-// densities of columns lx-2 and lx-1 gathered by the same lane just before
-// their second collision (lb.c:968-977); chunks are right-aligned so the
-// chunk holding lx also holds lx-2 and lx-1.  Unknown populations and the
-// ghost ring follow the collided-state convention of kernels_f64.hip: the
-// first step pulls them from P's ghost ring; the second step's are P(x)
-// itself; the output Q's ghost ring gets the first step's values.
-//
-// Storage S = double (bit-exact FP64) or float / _Float16 deviations (FP32
-// collisions, FP64 walls).  For reduced storage the first step's populations
-// are rounded to S exactly as a store would round them, so two steps here
-// equal two single steps bit for bit.
+// Storage S = double (bit-exact FP64) or float / _Float16 deviations (FP32:
+// FP64 collisions rounded stochastically; FP16: FP32 collisions; FP64 walls).
+// For reduced storage each intermediate step is rounded to S exactly as a
+// store would round it, so k steps here equal k single steps bit for bit.
 #include <utility>
 
 #include "launch.hpp"
