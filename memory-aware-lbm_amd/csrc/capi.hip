@@ -237,7 +237,7 @@ int step_impl(lbm_ctx *c, int variant, int nsteps) {
         // and a fused single step for the rest
         if (c->prob.lay.lx < 4)
             return fail(LBM_ERR_UNSUPPORTED, "two-step variants need lx >= 4");
-        if (!wpanel4_possible(c->prob.lay, c->tile_x)) return step_impl(c, LBM_PANEL, nsteps);
+        if (!wpanel4_possible(c->prob.lay, c->tile_x, c->storage)) return step_impl(c, LBM_PANEL, nsteps);
         if (!c->buf3) {
             size_t bytes = 9 * (size_t)c->prob.lay.plane * elem_size(c->storage);
             HIPCHK(hipMalloc(&c->buf3, bytes));

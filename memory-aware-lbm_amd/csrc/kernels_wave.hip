@@ -222,18 +222,25 @@ __global__ __launch_bounds__(64) void k_wpanel_edge(Problem p, WaveGrid g, uint3
 // arithmetic.
 constexpr uint32_t DROP = 0x80000000u;        // offset that no resource here reaches
 constexpr int RSRC_WORD3 = 0x00020000;        // raw buffer descriptor word 3 (CDNA)
-constexpr int CPOL_NT = 2;                    // non-temporal store
+// Stores go through L2 with the default (write-back) policy, not as
+// non-temporal streaming stores: a wave's owned rows (62 or 58 of them) do
+// not fill whole 128-byte lines, and a line shared by two waves is only
+// complete once both have written it.  Kept in L2, the two halves merge
+// before the line goes to HBM; streamed, each half reaches HBM as a partial
+// write.  On 16384 x 4096 the two-step FP64 sweep runs 5% faster this way,
+// FP32 13%, FP16 20%.
+constexpr int CPOL_STORE = 0;
 
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
 template <typename S>
 __device__ __forceinline__ void buffer_store(S v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
     if constexpr (sizeof(S) == 8)
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, off, 0, CPOL_NT);
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, off, 0, CPOL_STORE);
     else if constexpr (sizeof(S) == 4)
-        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, off, 0, CPOL_NT);
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, off, 0, CPOL_STORE);
     else
-        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(uint16_t, v), r, off, 0, CPOL_NT);
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(uint16_t, v), r, off, 0, CPOL_STORE);
 }
 
 template <typename S>
@@ -446,9 +453,16 @@ WaveGrid wave_grid(const Layout &L, int cx) { return wave_grid_max(L, cx, 16, 16
 // the arithmetic per column, so it wants wider chunks, up to 24, and fewer
 // waves (about 6144) before narrowing them: FP64 GLUPS at 4 / 8 / 12 / 24
 // columns: 4096 x 1024 56 / 65 / 66 / 51, 8192 x 2048 66 / 80 / 85 / 86,
-// 16384 x 4096 - / 91 / 97 / 100.
-static WaveGrid wave_grid4(const Layout &L, int cx) {
-    WaveGrid g = wave_grid_max(L, cx, 24, 6144);
+// 16384 x 4096 - / 91 / 97 / 100.  Only the largest lattices gain from
+// chunks wider still (fewer halo columns re-read): 16384 x 4096 at 24 / 40 /
+// 64 columns runs FP64 104 / 106 / 107, FP32 128 / 134 / 132, but FP16
+// 217 / 204 / 187, and 8192 x 2048 loses at 40 in every format (FP32 105 at
+// 24, 91 at 40); FP64 and FP32 take 40 columns where 24 would leave more
+// than 128 x 6144 waves.
+static WaveGrid wave_grid4(const Layout &L, int cx, int storage) {
+    const int64_t bands = (L.ly + WAVE_ROWS - 1) / WAVE_ROWS;
+    const int cmax = storage != 2 && bands * L.lx >= 128 * 6144 ? 40 : 24;
+    WaveGrid g = wave_grid_max(L, cx, cmax, 6144);
     constexpr int ROWS4 = 64 - 2 * 3;
     auto chunk_x0 = [&](int ck) { return L.lx + 1 - (g.ncx - 1 - ck) * g.cx - g.cx; };
     while (g.b_hi > g.b_lo && 1 + g.b_hi * WAVE_ROWS + 2 > L.ly - 1) --g.b_hi;
@@ -469,8 +483,8 @@ static WaveGrid wave_grid4(const Layout &L, int cx) {
     return g;
 }
 
-bool wpanel4_possible(const Layout &L, int cx) {
-    WaveGrid g = wave_grid4(L, cx);
+bool wpanel4_possible(const Layout &L, int cx, int storage) {
+    WaveGrid g = wave_grid4(L, cx, storage);
     // the widened frame of the first perimeter launch must leave an interior
     return g.b_hi - g.b_lo > 2 && g.c_hi - g.c_lo > 2;
 }
@@ -508,7 +522,7 @@ hipError_t launch_wpanel(int storage, const Problem &p, const WaveGrid &g, const
 hipError_t launch_wpanel4(int storage, const Problem &p, int cx, const void *P, void *S, void *Q,
                           hipStream_t st, hipStream_t side, hipEvent_t fork, hipEvent_t join) {
     const Layout &L = p.lay;
-    WaveGrid g = wave_grid4(L, cx);
+    WaveGrid g = wave_grid4(L, cx, storage);
     if (g.b_hi - g.b_lo <= 2 || g.c_hi - g.c_lo <= 2) return hipErrorInvalidValue;
     // frame of the first perimeter launch: one band / chunk wider, so that its
     // output in S covers every value the second launch pulls

@@ -80,7 +80,7 @@ hipError_t launch_wpanel(int storage, const Problem &p, const WaveGrid &g, const
 // (P -> Q); the perimeter frame runs two two-step launches through the third
 // buffer S (P -> S over the frame widened by one band / chunk, then S -> Q).
 // Returns hipErrorInvalidValue when the lattice has no interior for it.
-bool wpanel4_possible(const Layout &L, int cx);
+bool wpanel4_possible(const Layout &L, int cx, int storage);
 hipError_t launch_wpanel4(int storage, const Problem &p, int cx, const void *P, void *S, void *Q,
                           hipStream_t st, hipStream_t side, hipEvent_t fork, hipEvent_t join);
 

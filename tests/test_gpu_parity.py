@@ -162,11 +162,12 @@ def test_panel4_against_oracle(orc, geom):
     """Four steps per launch (three levels in registers, the perimeter frame
     through the third buffer) against the oracle, bit for bit, for step
     counts that leave every remainder (a two-step launch, a single fused
-    step) and for chunk widths with one and two columns in flight."""
+    step) and for chunk widths with one and two columns in flight, up to the
+    40 columns the largest lattices use."""
     ch = OChannel.geometry(600, 400, geom)
     for steps in (4, 5, 6, 7, 13):
         _, lref, tref, pref = orc.run(ch, steps)
-        for cxk in (0, 5, 8, 16):
+        for cxk in (0, 5, 8, 16, 40):
             with L.Lattice(dev_channel(ch)) as lat:
                 lat.set_tile(cxk, 0)
                 lat.init_equilibrium()
