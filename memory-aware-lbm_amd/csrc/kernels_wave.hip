@@ -321,16 +321,16 @@ __global__ __launch_bounds__(64 * WPG) void k_wpanel_in(Problem p, WaveGrid g,
     }
 
     // per population, a wave-uniform base (SGPRs) with pb[I][v(x)] the value
-    // pulled into (x, y), v(x) = (x - X0 + 3) * pitch + y + yoff: one 32-bit
+    // pulled into (x, y), v(x) = (x - X0 + H) * pitch + y + yoff: one 32-bit
     // lane offset per column serves all nine loads
     const S *pb[9];
     for_pops([&]<int I>() {
-        pb[I] = P + I * L.plane + (int64_t)(X0 - 3 - cx(I)) * L.pitch - cy(I);
+        pb[I] = P + I * L.plane + (int64_t)(X0 - H - cx(I)) * L.pitch - cy(I);
     });
     const uint32_t vy = (uint32_t)(yl + L.yoff) * (uint32_t)sizeof(S);
     auto pull = [&](int x, S (&dst)[9]) {
         // byte offset kept 32-bit so the loads take the SGPR-base form
-        const uint32_t v = (uint32_t)(x - X0 + 3) * colb + vy;
+        const uint32_t v = (uint32_t)(x - X0 + H) * colb + vy;
         for_pops([&]<int I>() {
             dst[I] = *reinterpret_cast<const S *>(reinterpret_cast<const char *>(pb[I]) + v);
         });
