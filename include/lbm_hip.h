@@ -143,6 +143,18 @@ int lbm_hip_step(lbm_ctx *ctx, int variant, int nsteps);
 int lbm_hip_collide(lbm_ctx *ctx);
 int lbm_hip_propagate(lbm_ctx *ctx);
 
+/* Periodic ghost copy, makePeriodic (lb.c:1783-1817), "to be called after
+ * the propagation step": every perimeter node receives, in the slots of the
+ * populations that arrive from outside the domain, what left the domain
+ * through the opposite side (corners from the opposite corner), exactly as
+ * the reference copies them out of the ghost ring its propagate pushed them
+ * into.  Acts on the post-stream state; a collided context (after
+ * lbm_hip_collide, or a fused or blocked step) is first brought to the
+ * post-stream state, as lbm_hip_propagate would, so a following
+ * lbm_hip_propagate returns LBM_ERR_STATE.  Every storage format.  Enqueued
+ * on the context's stream. */
+int lbm_hip_make_periodic(lbm_ctx *ctx);
+
 /* Block-size knobs of the blocked variants (the reference's blk_size /
  * thread_block, unsteady.c:93,258).  TIGHT_BLOCK: tile extent in x and y.
  * PANEL: tile_x = columns per wave's march (>= 3); tile_y is not used.

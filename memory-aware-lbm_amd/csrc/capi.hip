@@ -523,6 +523,16 @@ int lbm_hip_propagate(lbm_ctx *c) {
     return to_streamed(c);
 }
 
+int lbm_hip_make_periodic(lbm_ctx *c) {
+    if (!c) return fail(LBM_ERR_ARG, "null context");
+    // the copy acts on the post-stream state: L = buf[cur], T = buf[1 - cur]
+    int rc = to_streamed(c);
+    if (rc) return rc;
+    HIPCHK(launch_periodic(c->storage, c->prob.lay, c->buf[1 - c->cur], c->buf[c->cur],
+                           c->stream));
+    return LBM_OK;
+}
+
 int lbm_hip_set_graph_steps(lbm_ctx *c, int graph_steps) {
     if (!c || graph_steps < 0) return fail(LBM_ERR_ARG, "bad graph length");
     HIPCHK(hipStreamSynchronize(c->stream));

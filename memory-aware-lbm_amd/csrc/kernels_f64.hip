@@ -24,6 +24,7 @@
 //  k_stream          collided -> streamed: pull gather only.
 //  k_init            equilibrium initial condition.
 //  k_macros / k_diag moments and reductions of the post-stream state.
+//  k_periodic        the periodic ghost copy on a streamed state (any storage).
 #include "launch.hpp"
 #include "collide_f64.hpp"
 #include "fields.hpp"
@@ -173,6 +174,70 @@ __global__ void k_bump(uint32_t *ctr, uint32_t by) {
 
 hipError_t launch_bump(uint32_t *ctr, uint32_t by, hipStream_t st) {
     k_bump<<<1, 64, 0, st>>>(ctr, by);
+    return hipGetLastError();
+}
+
+// Periodic ghost copy (makePeriodic, lb.c:1783-1817) on a streamed state:
+// each perimeter node receives, in the slots of populations arriving from
+// outside, what left the domain through the opposite side.  The reference
+// copies those values from its ghost ring, where propagate (lb.c:258-267)
+// pushed them; here they are read at their origin in T, the post-collision
+// state the push came from: ghost slot i of (gx, gy) holds T_i(gx - c_i,
+// gy - c_i) when that node is interior, and 0 (the calloc'd buffer, in
+// deviation storage -w_i) when no interior node feeds it.  The row pass, the
+// column pass and the four corner diagonals write disjoint slots once the
+// four corner slots, which the reference writes three times, are left to
+// their last writer (the diagonal); that needs lx >= 2 and ly >= 2 (contexts
+// are at least 3 x 3).
+template <typename S> __device__ __forceinline__ S stored_zero(int i) {
+    if constexpr (sizeof(S) == 8) return 0.;
+    else return (S)(-(i == 0 ? 4. / 9. : (i <= 4 ? 1. / 9. : 1. / 36.)));
+}
+
+template <typename S>
+__global__ void k_periodic(Layout L, const S *__restrict__ T, S *__restrict__ Lb) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lx = L.lx, ly = L.ly;
+    auto at = [&](int i, int x, int y) { return (int64_t)i * L.plane + L.node(x, y); };
+    auto ghost = [&](int i, int gx, int gy) -> S {
+        const int sx = gx - cx(i), sy = gy - cy(i);
+        if (sx >= 1 && sx <= lx && sy >= 1 && sy <= ly) return T[at(i, sx, sy)];
+        return stored_zero<S>(i);
+    };
+    if (t < lx) {
+        const int x = t + 1;
+        Lb[at(4, x, ly)] = ghost(4, x, 0);
+        if (x != lx) Lb[at(7, x, ly)] = ghost(7, x, 0);
+        if (x != 1) Lb[at(8, x, ly)] = ghost(8, x, 0);
+        Lb[at(2, x, 1)] = ghost(2, x, ly + 1);
+        if (x != 1) Lb[at(5, x, 1)] = ghost(5, x, ly + 1);
+        if (x != lx) Lb[at(6, x, 1)] = ghost(6, x, ly + 1);
+    }
+    if (t < ly) {
+        const int y = t + 1;
+        Lb[at(1, 1, y)] = ghost(1, lx + 1, y);
+        if (y != 1) Lb[at(5, 1, y)] = ghost(5, lx + 1, y);
+        if (y != ly) Lb[at(8, 1, y)] = ghost(8, lx + 1, y);
+        Lb[at(3, lx, y)] = ghost(3, 0, y);
+        if (y != 1) Lb[at(6, lx, y)] = ghost(6, 0, y);
+        if (y != ly) Lb[at(7, lx, y)] = ghost(7, 0, y);
+    }
+    if (t == 0) {
+        Lb[at(5, 1, 1)] = ghost(5, lx + 1, ly + 1);
+        Lb[at(6, lx, 1)] = ghost(6, 0, ly + 1);
+        Lb[at(7, lx, ly)] = ghost(7, 0, 0);
+        Lb[at(8, 1, ly)] = ghost(8, lx + 1, 0);
+    }
+}
+
+hipError_t launch_periodic(int storage, const Layout &L, const void *T, void *Lb,
+                           hipStream_t st) {
+    if (L.lx < 2 || L.ly < 2) return hipErrorInvalidValue;
+    const int n = L.lx > L.ly ? L.lx : L.ly;
+    const int blocks = (n + 255) / 256;
+    if (storage == 0) k_periodic<double><<<blocks, 256, 0, st>>>(L, (const double *)T, (double *)Lb);
+    else if (storage == 1) k_periodic<float><<<blocks, 256, 0, st>>>(L, (const float *)T, (float *)Lb);
+    else k_periodic<_Float16><<<blocks, 256, 0, st>>>(L, (const _Float16 *)T, (_Float16 *)Lb);
     return hipGetLastError();
 }
 

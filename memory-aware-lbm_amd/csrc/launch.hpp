@@ -15,6 +15,9 @@ hipError_t launch_init_f64(const Layout &L, const double *init_ux, double *dst, 
 hipError_t launch_macros_f64(const Layout &L, bool pull, const double *src, double *rho,
                              double *ux, double *uy, hipStream_t st);
 hipError_t launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t st);
+// periodic ghost copy on a streamed state (L from T), any storage format
+hipError_t launch_periodic(int storage, const Layout &L, const void *T, void *Lb,
+                           hipStream_t st);
 hipError_t launch_bump(uint32_t *ctr, uint32_t by, hipStream_t st);
 int64_t diag_partials_needed(const Layout &L);
 hipError_t launch_diag_f64(const Problem &p, bool pull, const double *src, double *partial,

@@ -453,12 +453,18 @@ void collide_tight_panel_iy(Simulation *sim) {
 void collide_tight_panel_ix(Simulation *sim) { collide_tight_panel_iy(sim); }
 void collide_tight_panel_iy_openmp(Simulation *sim) { collide_tight_panel_iy(sim); }
 
-/* periodic ghost copy (lb.c:1783-1817): the reference never calls it; on a
- * bound lattice the device has no periodic mode. */
+/* periodic ghost copy (lb.c:1783-1817), after a propagation: on a bound
+ * lattice the device applies it to its post-stream state. */
 void makePeriodic(Simulation *sim) {
-    if (slot_of(sim) >= 0) {
-        fprintf(stderr, "makePeriodic: not available on a device-bound Simulation\n");
-        abort();
+    int k = slot_of(sim);
+    if (k >= 0) {
+        if (g_bound[k].mid_origin) {
+            fprintf(stderr, "makePeriodic: called between collide() and propagate(); "
+                            "it follows the propagation step\n");
+            abort();
+        }
+        check(lbm_hip_make_periodic(g_bound[k].ctx), "makePeriodic");
+        return;
     }
     int lx = sim->lx, ly = sim->ly;
     Node **lat = sim->lattice;

@@ -81,6 +81,7 @@ def library():
         "lbm_hip_apply_node_op": (C.c_int, [C.c_int, C.c_int, _P, _P, _P, C.c_double]),
         "lbm_hip_copy_bandwidth": (C.c_int, [C.c_size_t, C.c_int, _P]),
         "lbm_hip_set_graph_steps": (C.c_int, [vp, C.c_int]),
+        "lbm_hip_make_periodic": (C.c_int, [vp]),
         "lbm_hip_query": (C.c_int, [vp]),
         "lbm_hip_device_memory": (C.c_int, [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
         "lbm_hip_last_error": (C.c_char_p, []),
@@ -215,6 +216,10 @@ class Lattice:
 
     def step(self, n: int = 1, variant: str = "combine"):
         _check(library().lbm_hip_step(self._h, VARIANT[variant], int(n)), "step")
+
+    def make_periodic(self):
+        """makePeriodic (lb.c:1783-1817) on the post-stream state."""
+        _check(library().lbm_hip_make_periodic(self._h), "make_periodic")
 
     def set_tile(self, tx: int, ty: int):
         _check(library().lbm_hip_set_tile(self._h, tx, ty), "set_tile")

@@ -515,3 +515,34 @@ void orc_diag(const orc_cfg *cfg, const double *lat, double *out4) {
 #undef SOLID
     out4[0] = fx; out4[1] = fy; out4[2] = ke; out4[3] = mass;
 }
+
+/* Periodic ghost copy on a post-stream state (lb.c:1783-1817): what left the
+ * domain through one side, and was pushed into that side's ghost ring,
+ * re-enters at the opposite side.  Row pass, column pass, then the four
+ * corner diagonals, in the reference's order (a corner slot written by a
+ * pass is overwritten by its diagonal). */
+void orc_make_periodic(int lx, int ly, double *lat) {
+    const int ny = ly + 2;
+#define POP(x, y, i) lat[((size_t)(x) * ny + (y)) * 9 + (i)]
+    for (int x = 1; x <= lx; ++x) {
+        POP(x, ly, 4) = POP(x, 0, 4);          /* leaving downwards re-enters at the top */
+        POP(x, ly, 7) = POP(x, 0, 7);
+        POP(x, ly, 8) = POP(x, 0, 8);
+        POP(x, 1, 2) = POP(x, ly + 1, 2);      /* leaving upwards re-enters at the bottom */
+        POP(x, 1, 5) = POP(x, ly + 1, 5);
+        POP(x, 1, 6) = POP(x, ly + 1, 6);
+    }
+    for (int y = 1; y <= ly; ++y) {
+        POP(1, y, 1) = POP(lx + 1, y, 1);      /* leaving right re-enters on the left */
+        POP(1, y, 5) = POP(lx + 1, y, 5);
+        POP(1, y, 8) = POP(lx + 1, y, 8);
+        POP(lx, y, 3) = POP(0, y, 3);          /* leaving left re-enters on the right */
+        POP(lx, y, 6) = POP(0, y, 6);
+        POP(lx, y, 7) = POP(0, y, 7);
+    }
+    POP(1, 1, 5) = POP(lx + 1, ly + 1, 5);
+    POP(lx, 1, 6) = POP(0, ly + 1, 6);
+    POP(lx, ly, 7) = POP(0, 0, 7);
+    POP(1, ly, 8) = POP(lx + 1, 0, 8);
+#undef POP
+}

@@ -19,6 +19,9 @@
  *       bitwise golden lattices written by the real reference
  *       (tests/golden/traj_*.npz, c1_*.npz), plus the quiescent-channel and
  *       variant-equivalence properties.
+ *   orc_make_periodic: bitwise against the reference's makePeriodic on a
+ *       seeded random lattice and inside an origin trajectory
+ *       (tests/golden/periodic.npz).
  *   orc_diag (drag/lift by momentum exchange, kinetic energy): the reference
  *       has no such routine ("parity unpinned" against the reference); it is
  *       pinned by closed forms in tests/test_oracle.py instead.
@@ -73,6 +76,10 @@ int orc_apply_bc(int which, double *f, double p0, double p1, double omega);
  * out[1] = lift (y force), out[2] = kinetic energy sum(rho*|u|^2)/2 over
  * fluid nodes, out[3] = total mass over fluid nodes. */
 void orc_diag(const orc_cfg *cfg, const double *lat, double *out4);
+
+/* Periodic ghost copy (lb.c:1783-1817) on a lattice of the orc_run layout,
+ * in place. */
+void orc_make_periodic(int lx, int ly, double *lat);
 
 #ifdef __cplusplus
 }

@@ -155,7 +155,41 @@ double ref_omega(void) { return h_omega; }
  *   tmp_out: sim.tmpLattice, same shape
  *   prho_out: pressureBoundary[y].rho for y = 0..ly+1
  * Returns the number of steps actually run (two-step variants round up). */
+static int run_with(const ref_cfg *c, int periodic_after, double *lat_out, double *tmp_out,
+                    double *prho_out);
+
 int ref_run(const ref_cfg *c, double *lat_out, double *tmp_out, double *prho_out) {
+    return run_with(c, -1, lat_out, tmp_out, prho_out);
+}
+
+/* As ref_run, with the reference's makePeriodic (lb.c:1783-1817) applied to
+ * sim.lattice once, right after the step that brings the count to
+ * periodic_after (one-step variants only: it is meant to follow a
+ * propagation). */
+int ref_run_periodic(const ref_cfg *c, int periodic_after, double *lat_out, double *tmp_out,
+                     double *prho_out) {
+    if (c->variant > 1 || periodic_after < 1) return -1;
+    return run_with(c, periodic_after, lat_out, tmp_out, prho_out);
+}
+
+/* The reference's makePeriodic on one lattice in the copy_out layout. */
+void ref_make_periodic(int lx, int ly, double *lat) {
+    Simulation s;
+    constructSim(&s, lx, ly);
+    size_t k = 0;
+    for (int x = 0; x < lx + 2; ++x)
+        for (int y = 0; y < ly + 2; ++y)
+            for (int i = 0; i < 9; ++i) s.lattice[x][y].fPop[i] = lat[k++];
+    makePeriodic(&s);
+    k = 0;
+    for (int x = 0; x < lx + 2; ++x)
+        for (int y = 0; y < ly + 2; ++y)
+            for (int i = 0; i < 9; ++i) lat[k++] = s.lattice[x][y].fPop[i];
+    destructSim(&s);
+}
+
+static int run_with(const ref_cfg *c, int periodic_after, double *lat_out, double *tmp_out,
+                    double *prho_out) {
     void (*coll)(Simulation *) = NULL;
     void (*strm)(Simulation *) = NULL;
     int twostep = 0;
@@ -187,6 +221,7 @@ int ref_run(const ref_cfg *c, double *lat_out, double *tmp_out, double *prho_out
         coll(&h_sim);
         if (!twostep) strm(&h_sim);
         ran += twostep ? 2 : 1;
+        if (ran == periodic_after) makePeriodic(&h_sim);
     }
     if (lat_out) copy_out(h_sim.lattice, lat_out);
     if (tmp_out) copy_out(h_sim.tmpLattice, tmp_out);

@@ -25,6 +25,11 @@ Fixtures
                  density and velocity fields by computeMacros.
   variants.json  serial origin/combine/tight/tight_block/panel bit-equality
                  on the parity lattices, and the OpenMP flavours' agreement.
+  periodic.npz   makePeriodic (lb.c:1783-1817) on a seeded random 11x8
+                 lattice, and a 32x16 origin trajectory of 12 steps with
+                 makePeriodic applied after step 5.
+
+`python tests/golden/make_golden.py periodic` writes only periodic.npz.
 """
 from __future__ import annotations
 
@@ -145,11 +150,28 @@ def make_variants(ref, c1):
     return rec
 
 
+def make_periodic(ref):
+    rng = np.random.default_rng(SEED + 1)
+    lx, ly = 11, 8
+    lat_in = rng.standard_normal((lx + 2, ly + 2, 9))
+    lat_out = ref.make_periodic(lx, ly, lat_in)
+    ch = Channel.geometry(32, 16, "A")
+    ran, lat, tmp, prho = ref.run_periodic(ch, 12, 5, "origin")
+    assert ran == 12
+    np.savez_compressed(os.path.join(HERE, "periodic.npz"), kat_lx=lx, kat_ly=ly,
+                        kat_in=lat_in, kat_out=lat_out, traj_steps=12, traj_after=5,
+                        lat=lat, tmp=tmp, prho=prho, channel=np.array(json.dumps(ch.__dict__)))
+
+
 def main():
     ref = reference()
     if not ref.available:
         sys.exit("oracle/_ref/libref_harness.so is missing: run `make -C oracle ref` "
                  "where the reference checkout exists")
+    if sys.argv[1:] == ["periodic"]:
+        make_periodic(ref)
+        return
+    make_periodic(ref)
     make_kat(ref)
     make_traj(ref, Channel.geometry(32, 16, "A"), "traj_32x16_reg.npz", TRAJ_STEPS)
     make_traj(ref, Channel.geometry(32, 16, "A", bc_family=1), "traj_32x16_zh.npz",

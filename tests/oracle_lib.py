@@ -96,7 +96,13 @@ def _load(path: str):
     a = getattr(lib, f"{prefix}_apply_bc")
     a.argtypes = [C.c_int, _P, C.c_double, C.c_double, C.c_double]
     a.restype = C.c_int
+    mp = getattr(lib, f"{prefix}_make_periodic")
+    mp.argtypes = [C.c_int, C.c_int, _P]
+    mp.restype = None
     lib._prefix = prefix
+    if prefix == "ref":
+        lib.ref_run_periodic.argtypes = [C.POINTER(_Cfg), C.c_int, _P, _P, _P]
+        lib.ref_run_periodic.restype = C.c_int
     if prefix == "orc":
         lib.orc_run_from.argtypes = [C.POINTER(_Cfg), _P, _P, _P, _P, _P, _P]
         lib.orc_run_from.restype = C.c_int
@@ -154,6 +160,32 @@ class Impl:
         if ran < 0:
             raise RuntimeError("orc_run_from failed")
         return ran, lo, to, po
+
+    def make_periodic(self, lx: int, ly: int, lat) -> np.ndarray:
+        """makePeriodic (lb.c:1783-1817) on a copy of a (lx+2, ly+2, 9) lattice."""
+        out = np.array(lat, dtype=np.float64, order="C")
+        assert out.shape == (lx + 2, ly + 2, 9)
+        self._fn("make_periodic")(int(lx), int(ly), out.ctypes.data_as(_P))
+        return out
+
+    def run_periodic(self, ch: Channel, steps: int, after: int, variant: str = "origin"):
+        """steps steps with makePeriodic applied once after step `after`:
+        the reference harness runs it inside its own time loop, the oracle
+        as run(after) + make_periodic + run_from(steps - after)."""
+        if self.lib._prefix == "ref":
+            cfg = self._cfg(ch, variant, steps)
+            lat = np.zeros(ch.state_shape())
+            tmp = np.zeros(ch.state_shape())
+            prho = np.zeros(ch.ly + 2)
+            ran = self.lib.ref_run_periodic(C.byref(cfg), int(after), lat.ctypes.data_as(_P),
+                                            tmp.ctypes.data_as(_P), prho.ctypes.data_as(_P))
+            if ran < 0:
+                raise RuntimeError("ref_run_periodic failed")
+            return ran, lat, tmp, prho
+        r1, lat, tmp, prho = self.run(ch, after, variant)
+        lat = self.make_periodic(ch.lx, ch.ly, lat)
+        r2, lat, tmp, prho = self.run_from(ch, steps - after, lat, tmp, prho, variant)
+        return r1 + r2, lat, tmp, prho
 
     def diag(self, ch: Channel, lat) -> np.ndarray:
         cfg = self._cfg(ch, "origin", 0)

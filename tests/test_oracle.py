@@ -58,6 +58,54 @@ def test_trajectories_bitwise(orc, name):
         assert np.array_equal(prho, g[f"prho_{s}"]), (name, s)
 
 
+def test_make_periodic_against_reference_fixture(orc):
+    """makePeriodic (lb.c:1783-1817): on a seeded random lattice, and inside
+    a 12-step origin trajectory with the copy after step 5, bit for bit
+    against what the real reference wrote (ghost rings included)."""
+    g = np.load(os.path.join(GOLD, "periodic.npz"))
+    lx, ly = int(g["kat_lx"]), int(g["kat_ly"])
+    out = orc.make_periodic(lx, ly, g["kat_in"])
+    assert np.array_equal(out, g["kat_out"])
+    # every slot the copy writes lies on the perimeter rows / columns
+    changed = np.argwhere(out != g["kat_in"])
+    assert all(x in (1, lx) or y in (1, ly) for x, y, _ in changed)
+    ch = Channel(**json.loads(str(g["channel"])))
+    ran, lat, tmp, prho = orc.run_periodic(ch, int(g["traj_steps"]), int(g["traj_after"]))
+    assert ran == int(g["traj_steps"])
+    assert np.array_equal(lat, g["lat"]) and np.array_equal(tmp, g["tmp"])
+    assert np.array_equal(prho, g["prho"])
+    # the copy changes the flow (a check that the fixture exercises it)
+    assert not np.array_equal(orc.run(ch, ran, "origin")[1], g["lat"])
+
+
+def test_make_periodic_wraps_each_leaving_population(orc):
+    """Closed form of the copy: a population that left through one side
+    (and sits in that side's ghost ring) is what the opposite perimeter node
+    receives, for every side and corner; the ghost ring itself is untouched."""
+    lx, ly = 6, 5
+    lat = np.random.default_rng(9).standard_normal((lx + 2, ly + 2, 9))
+    out = orc.make_periodic(lx, ly, lat)
+    for x in range(1, lx + 1):
+        for i in (4, 7, 8):
+            if (x, i) not in ((lx, 7), (1, 8)):
+                assert out[x, ly, i] == lat[x, 0, i]
+        for i in (2, 5, 6):
+            if (x, i) not in ((1, 5), (lx, 6)):
+                assert out[x, 1, i] == lat[x, ly + 1, i]
+    for y in range(1, ly + 1):
+        for i in (1, 5, 8):
+            if (y, i) not in ((1, 5), (ly, 8)):
+                assert out[1, y, i] == lat[lx + 1, y, i]
+        for i in (3, 6, 7):
+            if (y, i) not in ((1, 6), (ly, 7)):
+                assert out[lx, y, i] == lat[0, y, i]
+    assert out[1, 1, 5] == lat[lx + 1, ly + 1, 5] and out[lx, 1, 6] == lat[0, ly + 1, 6]
+    assert out[lx, ly, 7] == lat[0, 0, 7] and out[1, ly, 8] == lat[lx + 1, 0, 8]
+    ring = np.ones((lx + 2, ly + 2), bool)
+    ring[1:-1, 1:-1] = False
+    assert np.array_equal(out[ring], lat[ring])
+
+
 @pytest.mark.slow
 @pytest.mark.parametrize("geom", ["A", "B"])
 def test_c1_hashes(orc, geom):
