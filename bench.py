@@ -40,13 +40,14 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "GLUPS & % of HBM roofline (FP64 vs FP32-store); serving p50/p99 latency"
 KERNEL_OF = {"origin": "k_collide<false>", "combine": "k_collide<true>",
              "tight": "k_collide<true>", "tight_block": "k_block<64>",
-             "panel": "k_wpanel_in<double, 2, 2>", "panel4": "k_wpanel_in<double, 2, 4>"}
+             "panel": "k_wpanel_in<double, 2, 2>", "panel4": "k_wpanel_in<double, 1, 4>",
+             "panel6": "k_wpanel_in<double, 2, 6>"}
 LOWP_KERNEL_OF = {(s, v): k.replace("double", {"f32": "float", "f16": "_Float16"}[s])
                   for s in ("f32", "f16") for v, k in
                   (("combine", "k_collide_lp<double, true>"), ("panel", "k_wpanel_in<double, 2, 2>"),
                    ("panel4", "k_wpanel_in<double, 2, 4>"))}
 UPDATES_PER_LAUNCH = {"origin": 0.5, "combine": 1, "tight": 1, "tight_block": 2, "panel": 2,
-                      "panel4": 4}
+                      "panel4": 4, "panel6": 6}
 PROFILE = os.path.join(ROOT, "profiles", "kernels_16384x4096.md")
 
 
@@ -58,7 +59,7 @@ def parse():
     ap.add_argument("--lx", type=int, default=16384)
     ap.add_argument("--ly", type=int, default=4096)
     ap.add_argument("--variant", default="panel4")
-    ap.add_argument("--variants", default="origin,combine,tight_block,panel",
+    ap.add_argument("--variants", default="origin,combine,tight_block,panel,panel6",
                     help="variants timed besides the headline one (comma list, or '')")
     ap.add_argument("--other-steps", type=int, default=1000)
     ap.add_argument("--storages", default="f32,f16",
@@ -94,7 +95,8 @@ def time_variant(lbm, ch, variant, steps, warmup, storage="f64"):
 
 # entry of the profile's traffic line holding one launch of a variant (the
 # panel sweeps' interior and perimeter dispatches added up)
-TRAFFIC_KEY = {"panel": "panel_launch<double>", "panel4": "panel4_launch<double>"}
+TRAFFIC_KEY = {"panel": "panel_launch<double>", "panel4": "panel4_launch<double>",
+               "panel6": "panel6_launch<double>"}
 
 
 def measured_traffic(variant):

@@ -52,6 +52,10 @@ enum {
  *               held in registers (no counterpart in the reference; the same
  *               per-node operations in the same order).  Uses a third
  *               population buffer; lattices too small for it run as PANEL.
+ *   PANEL6      six time steps per launch, five levels in registers (no
+ *               counterpart in the reference either).  Uses a third and a
+ *               fourth population buffer; lattices too small for it run as
+ *               PANEL4.
  * All of them give bit-identical FP64 results, as the reference's serial
  * sweeps do. */
 typedef enum {
@@ -60,7 +64,8 @@ typedef enum {
     LBM_TIGHT = 2,
     LBM_TIGHT_BLOCK = 3,
     LBM_PANEL = 4,
-    LBM_PANEL4 = 5
+    LBM_PANEL4 = 5,
+    LBM_PANEL6 = 6
 } lbm_variant;
 
 /* Population storage.  F64 is the reference's arithmetic.  F32/F16 store the

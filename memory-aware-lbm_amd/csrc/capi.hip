@@ -38,7 +38,7 @@ struct lbm_ctx {
     size_t footprint;
     int graph_steps;               // fused steps per captured graph (0: eager)
     uint32_t *d_step;              // collision counter (keys FP32 stochastic rounding)
-    void *buf3;                    // third population buffer of the four-step sweep
+    void *buf3, *buf4;             // further population buffers of the four- and six-step sweeps
     hipStream_t side;              // second stream for the panel's edge waves
     hipEvent_t fork, join;
     hipGraphExec_t graph[2];       // fused steps, by starting buffer
@@ -143,12 +143,24 @@ hipError_t panel_launch(lbm_ctx *c, int from) {
     return e;
 }
 
-// one four-step launch from buffer `from` into 1 - from (third buffer buf3)
-hipError_t panel4_launch(lbm_ctx *c, int from) {
-    hipError_t e = launch_wpanel4(c->storage, c->prob, c->tile_x, c->buf[from], c->buf3,
-                                  c->buf[1 - from], c->stream, c->side, c->fork, c->join);
-    if (e == hipSuccess && c->storage != LBM_STORE_F64) e = launch_bump(c->d_step, 4, c->stream);
+// one launch of `levels` (4 or 6) steps from buffer `from` into 1 - from,
+// the perimeter frame passing through buf3 (and buf4)
+hipError_t deep_launch(lbm_ctx *c, int from, int levels) {
+    void *mid[2] = {c->buf3, c->buf4};
+    hipError_t e = launch_wpanel_deep(c->storage, c->prob, c->tile_x, levels, c->buf[from], mid,
+                                      c->buf[1 - from], c->stream, c->side, c->fork, c->join);
+    if (e == hipSuccess && c->storage != LBM_STORE_F64)
+        e = launch_bump(c->d_step, (uint32_t)levels, c->stream);
     return e;
+}
+
+// allocate a further population buffer on first use (counted in the footprint)
+int extra_buffer(lbm_ctx *c, void **b) {
+    if (*b) return LBM_OK;
+    size_t bytes = 9 * (size_t)c->prob.lay.plane * elem_size(c->storage);
+    HIPCHK(hipMalloc(b, bytes));
+    c->footprint += bytes;
+    return LBM_OK;
 }
 
 // capture graph_steps / 2 panel launches starting from buffer `start` (the
@@ -232,29 +244,31 @@ int step_impl(lbm_ctx *c, int variant, int nsteps) {
         }
         return LBM_OK;
     }
-    case LBM_PANEL4: {
-        // four steps per launch while four remain, then the two-step panel
-        // and a fused single step for the rest
+    case LBM_PANEL4:
+    case LBM_PANEL6: {
+        // four (six) steps per launch while four (six) remain; the rest with
+        // the shallower sweeps
         if (c->prob.lay.lx < 4)
             return fail(LBM_ERR_UNSUPPORTED, "two-step variants need lx >= 4");
-        if (!wpanel4_possible(c->prob.lay, c->tile_x, c->storage)) return step_impl(c, LBM_PANEL, nsteps);
-        if (!c->buf3) {
-            size_t bytes = 9 * (size_t)c->prob.lay.plane * elem_size(c->storage);
-            HIPCHK(hipMalloc(&c->buf3, bytes));
-            c->footprint += bytes;
-        }
+        const int levels = variant == LBM_PANEL6 ? 6 : 4;
+        const int fallback = variant == LBM_PANEL6 ? LBM_PANEL4 : LBM_PANEL;
+        if (!wpanel_deep_possible(c->prob.lay, c->tile_x, c->storage, levels))
+            return step_impl(c, fallback, nsteps);
+        int rc = extra_buffer(c, &c->buf3);
+        if (!rc && levels == 6) rc = extra_buffer(c, &c->buf4);
+        if (rc) return rc;
         int k = 0;
         if (!c->collided) {
-            int rc = to_collided(c);
+            rc = to_collided(c);
             if (rc) return rc;
             k = 1;
         }
-        for (; k + 4 <= nsteps; k += 4) {
-            HIPCHK(panel4_launch(c, c->cur));
+        for (; k + levels <= nsteps; k += levels) {
+            HIPCHK(deep_launch(c, c->cur, levels));
             c->cur = 1 - c->cur;
-            c->steps += 4;
+            c->steps += levels;
         }
-        return k < nsteps ? step_impl(c, LBM_PANEL, nsteps - k) : LBM_OK;
+        return k < nsteps ? step_impl(c, fallback, nsteps - k) : LBM_OK;
     }
     case LBM_TIGHT_BLOCK:
     case LBM_PANEL: {
@@ -430,6 +444,7 @@ int lbm_hip_destroy(lbm_ctx *c) {
     if (c->d_out4) (void)hipFree(c->d_out4);
     if (c->d_step) (void)hipFree(c->d_step);
     if (c->buf3) (void)hipFree(c->buf3);
+    if (c->buf4) (void)hipFree(c->buf4);
     if (c->side) (void)hipStreamSynchronize(c->side);
     if (c->fork) (void)hipEventDestroy(c->fork);
     if (c->join) (void)hipEventDestroy(c->join);
@@ -651,6 +666,7 @@ double lbm_hip_bytes_per_node(const lbm_ctx *c, int variant) {
     case LBM_TIGHT_BLOCK:
     case LBM_PANEL: return pop;             // one read and one write per two steps
     case LBM_PANEL4: return pop / 2;        // per four steps
+    case LBM_PANEL6: return pop / 3;        // per six steps
     default: return 0.0;
     }
 }

@@ -446,28 +446,36 @@ static WaveGrid wave_grid_max(const Layout &L, int cx, int cmax, int64_t waves) 
 
 WaveGrid wave_grid(const Layout &L, int cx) { return wave_grid_max(L, cx, 16, 16384); }
 
-// The grid of the four-step sweep: the interior rectangle of the two-step
-// grid, shrunk until its three-node halo stays off the perimeter (rows and
-// columns 2 .. ly-1 / lx-1), tiled by 58-row bands (the last one partial).
-// The four-step march recomputes six halo columns per chunk and does twice
-// the arithmetic per column, so it wants wider chunks, up to 24, and fewer
-// waves (about 6144) before narrowing them: FP64 GLUPS at 4 / 8 / 12 / 24
-// columns: 4096 x 1024 56 / 65 / 66 / 51, 8192 x 2048 66 / 80 / 85 / 86,
-// 16384 x 4096 - / 91 / 97 / 100.  Only the largest lattices gain from
-// chunks wider still (fewer halo columns re-read): 16384 x 4096 at 24 / 40 /
-// 64 columns runs FP64 104 / 106 / 107, FP32 128 / 134 / 132, but FP16
+// The grid of the deep sweeps (LEVELS = 4 or 6 steps per launch, H = LEVELS-1
+// halo rows and columns): the interior rectangle of the two-step grid, shrunk
+// until its H-node halo stays off the perimeter (rows and columns 2 .. ly-1 /
+// lx-1), tiled by (64 - 2H)-row bands (the last one partial).  The deep
+// march recomputes 2H halo columns per chunk and does LEVELS/2 times the
+// two-step arithmetic per column, so it wants wider chunks, up to 24, and
+// fewer waves (about 6144) before narrowing them: four-step FP64 GLUPS at 4 /
+// 8 / 12 / 24 columns: 4096 x 1024 56 / 65 / 66 / 51, 8192 x 2048 66 / 80 /
+// 85 / 86, 16384 x 4096 - / 91 / 97 / 100.  Only the largest lattices gain
+// from chunks wider still (fewer halo columns re-read): 16384 x 4096 at 24 /
+// 40 / 64 columns runs FP64 104 / 106 / 107, FP32 128 / 134 / 132, but FP16
 // 217 / 204 / 187, and 8192 x 2048 loses at 40 in every format (FP32 105 at
-// 24, 91 at 40); FP64 and FP32 take 40 columns where 24 would leave more
-// than 128 x 6144 waves.
-static WaveGrid wave_grid4(const Layout &L, int cx, int storage) {
+// 24, 91 at 40); four steps take 40 columns in FP64 and FP32 where 24 would
+// leave more than 128 x 6144 waves.  There, FP64 runs one column of loads in
+// flight better than two (odd widths: 39 / 47 / 55 / 63 columns 106.7 /
+// 107.2 / 107.5 / 107.7 GLUPS against 105.7 / 105.6 / 104.9 / 106.0 at 40 /
+// 48 / 56 / 64) and takes 63.  Six steps (ten halo columns per chunk) take
+// 64: FP64 101 / 101 / 106 at 47 / 55 / 64.
+static WaveGrid wave_grid_deep(const Layout &L, int cx, int storage, int levels) {
     const int64_t bands = (L.ly + WAVE_ROWS - 1) / WAVE_ROWS;
-    const int cmax = storage != 2 && bands * L.lx >= 128 * 6144 ? 40 : 24;
+    const bool largest = bands * L.lx >= 128 * 6144;
+    const int cmax = !largest ? 24 : levels == 6 ? 64 : storage != 2 ? 40 : 24;
+    if (cx < 3 && largest && levels == 4 && storage == 0) cx = 63;
     WaveGrid g = wave_grid_max(L, cx, cmax, 6144);
-    constexpr int ROWS4 = 64 - 2 * 3;
+    const int H = levels - 1, rows = 64 - 2 * H;
     auto chunk_x0 = [&](int ck) { return L.lx + 1 - (g.ncx - 1 - ck) * g.cx - g.cx; };
-    while (g.b_hi > g.b_lo && 1 + g.b_hi * WAVE_ROWS + 2 > L.ly - 1) --g.b_hi;
-    while (g.c_lo < g.c_hi && chunk_x0(g.c_lo) - 3 < 2) ++g.c_lo;
-    while (g.c_hi > g.c_lo && chunk_x0(g.c_hi - 1) + g.cx + 2 > L.lx - 1) --g.c_hi;
+    while (g.b_hi > g.b_lo && 1 + g.b_hi * WAVE_ROWS - 1 + H > L.ly - 1) --g.b_hi;
+    while (g.b_lo < g.b_hi && 1 + g.b_lo * WAVE_ROWS - H < 2) ++g.b_lo;
+    while (g.c_lo < g.c_hi && chunk_x0(g.c_lo) - H < 2) ++g.c_lo;
+    while (g.c_hi > g.c_lo && chunk_x0(g.c_hi - 1) + g.cx - 1 + H > L.lx - 1) --g.c_hi;
     if (g.b_lo >= g.b_hi || g.c_lo >= g.c_hi) {
         g.b_lo = g.b_hi = g.c_lo = g.c_hi = 0;
         g.nin = 0;
@@ -477,16 +485,19 @@ static WaveGrid wave_grid4(const Layout &L, int cx, int storage) {
     }
     g.r0 = 1 + g.b_lo * WAVE_ROWS;
     g.r1 = 1 + g.b_hi * WAVE_ROWS;
-    g.nbl = (g.r1 - g.r0 + ROWS4 - 1) / ROWS4;
+    g.nbl = (g.r1 - g.r0 + rows - 1) / rows;
     g.nin = g.nbl * (g.c_hi - g.c_lo);
     g.nedge = g.ntasks - (g.b_hi - g.b_lo) * (g.c_hi - g.c_lo);
     return g;
 }
 
-bool wpanel4_possible(const Layout &L, int cx, int storage) {
-    WaveGrid g = wave_grid4(L, cx, storage);
-    // the widened frame of the first perimeter launch must leave an interior
-    return g.b_hi - g.b_lo > 2 && g.c_hi - g.c_lo > 2;
+bool wpanel_deep_possible(const Layout &L, int cx, int storage, int levels) {
+    if (levels != 4 && levels != 6) return false;
+    WaveGrid g = wave_grid_deep(L, cx, storage, levels);
+    // the frames of the perimeter launches, widened by up to LEVELS/2 - 1
+    // bands / chunks, must leave an interior
+    const int w = levels / 2 - 1;
+    return g.b_hi - g.b_lo > 2 * w && g.c_hi - g.c_lo > 2 * w;
 }
 
 // Perimeter waves on `side`, interior waves on `st`, joined back into `st`:
@@ -519,16 +530,12 @@ hipError_t launch_wpanel(int storage, const Problem &p, const WaveGrid &g, const
     return e;
 }
 
-hipError_t launch_wpanel4(int storage, const Problem &p, int cx, const void *P, void *S, void *Q,
-                          hipStream_t st, hipStream_t side, hipEvent_t fork, hipEvent_t join) {
-    const Layout &L = p.lay;
-    WaveGrid g = wave_grid4(L, cx, storage);
-    if (g.b_hi - g.b_lo <= 2 || g.c_hi - g.c_lo <= 2) return hipErrorInvalidValue;
-    // frame of the first perimeter launch: one band / chunk wider, so that its
-    // output in S covers every value the second launch pulls
-    WaveGrid ga = g;
-    ga.b_lo += 1; ga.b_hi -= 1; ga.c_lo += 1; ga.c_hi -= 1;
-    ga.nedge = ga.ntasks - (ga.b_hi - ga.b_lo) * (ga.c_hi - ga.c_lo);
+hipError_t launch_wpanel_deep(int storage, const Problem &p, int cx, int levels, const void *P,
+                              void *const *mid, void *Q, hipStream_t st, hipStream_t side,
+                              hipEvent_t fork, hipEvent_t join) {
+    if (!wpanel_deep_possible(p.lay, cx, storage, levels)) return hipErrorInvalidValue;
+    const WaveGrid g = wave_grid_deep(p.lay, cx, storage, levels);
+    const int nmid = levels / 2 - 1;
     const int depth = g.cx % 2 == 0 ? 2 : 1;
     const int blocks = (g.nin + WPG - 1) / WPG;
     hipError_t e = hipEventRecord(fork, st);
@@ -536,10 +543,27 @@ hipError_t launch_wpanel4(int storage, const Problem &p, int cx, const void *P, 
     if (e != hipSuccess) return e;
     auto go = [&](auto tag) {
         using T = decltype(tag);
-        k_wpanel_edge<T><<<ga.nedge, 64, 0, side>>>(p, ga, 0u, (const T *)P, (T *)S);
-        k_wpanel_edge<T><<<g.nedge, 64, 0, side>>>(p, g, 2u, (const T *)S, (T *)Q);
-        if (depth == 2) k_wpanel_in<T, 2, 4><<<blocks, 64 * WPG, 0, st>>>(p, g, (const T *)P, (T *)Q);
-        else k_wpanel_in<T, 1, 4><<<blocks, 64 * WPG, 0, st>>>(p, g, (const T *)P, (T *)Q);
+        // perimeter launch j advances the frame by steps 2j+1, 2j+2, from P or
+        // mid[j-1] into mid[j] or Q, over the frame widened by nmid - j bands /
+        // chunks, so that its output covers every value launch j+1 pulls
+        for (int j = 0; j <= nmid; ++j) {
+            WaveGrid gj = g;
+            const int w = nmid - j;
+            gj.b_lo += w; gj.b_hi -= w; gj.c_lo += w; gj.c_hi -= w;
+            gj.nedge = gj.ntasks - (gj.b_hi - gj.b_lo) * (gj.c_hi - gj.c_lo);
+            const T *src = (const T *)(j == 0 ? P : mid[j - 1]);
+            T *dst = (T *)(j == nmid ? Q : mid[j]);
+            k_wpanel_edge<T><<<gj.nedge, 64, 0, side>>>(p, gj, 2u * j, src, dst);
+        }
+        const T *src = (const T *)P;
+        T *dst = (T *)Q;
+        if (levels == 4) {
+            if (depth == 2) k_wpanel_in<T, 2, 4><<<blocks, 64 * WPG, 0, st>>>(p, g, src, dst);
+            else k_wpanel_in<T, 1, 4><<<blocks, 64 * WPG, 0, st>>>(p, g, src, dst);
+        } else {
+            if (depth == 2) k_wpanel_in<T, 2, 6><<<blocks, 64 * WPG, 0, st>>>(p, g, src, dst);
+            else k_wpanel_in<T, 1, 6><<<blocks, 64 * WPG, 0, st>>>(p, g, src, dst);
+        }
     };
     if (storage == 0) go(double{});
     else if (storage == 1) go(float{});

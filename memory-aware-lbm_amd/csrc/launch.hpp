@@ -79,12 +79,14 @@ __host__ __device__ inline WaveSpan wave_span(const Layout &L, const WaveGrid &g
 WaveGrid wave_grid(const Layout &L, int cx);
 hipError_t launch_wpanel(int storage, const Problem &p, const WaveGrid &g, const void *P, void *Q,
                          hipStream_t st, hipStream_t side, hipEvent_t fork, hipEvent_t join);
-// Four steps per launch: the interior waves march four levels in registers
-// (P -> Q); the perimeter frame runs two two-step launches through the third
-// buffer S (P -> S over the frame widened by one band / chunk, then S -> Q).
+// LEVELS = 4 or 6 steps per launch: the interior waves march LEVELS levels
+// in registers (P -> Q); the perimeter frame runs LEVELS/2 two-step launches
+// through LEVELS/2 - 1 further buffers mid[] (P -> mid[0] over the frame
+// widened by LEVELS/2 - 1 bands / chunks, ..., the last one into Q).
 // Returns hipErrorInvalidValue when the lattice has no interior for it.
-bool wpanel4_possible(const Layout &L, int cx, int storage);
-hipError_t launch_wpanel4(int storage, const Problem &p, int cx, const void *P, void *S, void *Q,
-                          hipStream_t st, hipStream_t side, hipEvent_t fork, hipEvent_t join);
+bool wpanel_deep_possible(const Layout &L, int cx, int storage, int levels);
+hipError_t launch_wpanel_deep(int storage, const Problem &p, int cx, int levels, const void *P,
+                              void *const *mid, void *Q, hipStream_t st, hipStream_t side,
+                              hipEvent_t fork, hipEvent_t join);
 
 }  // namespace lbm

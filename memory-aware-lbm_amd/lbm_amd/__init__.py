@@ -22,9 +22,10 @@ import numpy as np
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_DIR, "liblbm_hip.so")
 
-ORIGIN, COMBINE, TIGHT, TIGHT_BLOCK, PANEL, PANEL4 = range(6)
+ORIGIN, COMBINE, TIGHT, TIGHT_BLOCK, PANEL, PANEL4, PANEL6 = range(7)
 VARIANT = {"origin": ORIGIN, "combine": COMBINE, "tight": TIGHT,
-           "tight_block": TIGHT_BLOCK, "panel": PANEL, "panel4": PANEL4}
+           "tight_block": TIGHT_BLOCK, "panel": PANEL, "panel4": PANEL4,
+           "panel6": PANEL6}
 STORE_F64, STORE_F32, STORE_F16 = range(3)
 STORAGE = {"f64": STORE_F64, "f32": STORE_F32, "f16": STORE_F16}
 BC_REGULARIZED, BC_ZOUHE = 0, 1

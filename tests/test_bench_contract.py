@@ -19,11 +19,11 @@ def load_bench():
 
 def test_panel_traffic_from_profile():
     b = load_bench()
-    # interior + perimeter dispatches of one two- or four-step launch on
+    # interior + perimeter dispatches of one two-, four- or six-step launch on
     # 16384 x 4096: at least the algorithmic 144 B per node (one read and one
     # write of nine FP64 populations), at most 1.5x that
     alg = 144.0 * 16384 * 4096
-    for v in ("panel", "panel4"):
+    for v in ("panel", "panel4", "panel6"):
         t = b.measured_traffic(v)
         assert t is not None and alg <= t <= 1.5 * alg, (v, t)
     assert b.measured_traffic("tight_block") > alg

@@ -206,6 +206,63 @@ def test_panel4_reduced_equals_single_steps(storage):
     assert np.array_equal(la, lb) and np.array_equal(ta, tb) and np.array_equal(pa, pb)
 
 
+@pytest.mark.parametrize("geom", ["A", "B"])
+def test_panel6_against_oracle(orc, geom):
+    """Six steps per launch (five levels in registers, the perimeter frame
+    through two further buffers) against the oracle, bit for bit, for step
+    counts that leave every remainder (a four-step, a two-step launch, a
+    single fused step) and for chunk widths with one and two columns in
+    flight, up to the 64 columns the largest lattices use."""
+    ch = OChannel.geometry(600, 400, geom)
+    for steps in (6, 7, 8, 9, 13, 19):
+        _, lref, tref, pref = orc.run(ch, steps)
+        for cxk in (0, 5, 8, 16, 64):
+            with L.Lattice(dev_channel(ch)) as lat:
+                lat.set_tile(cxk, 0)
+                lat.init_equilibrium()
+                lat.step(steps, "panel6")
+                lo, to, po = lat.download()
+            what = f"panel6 {geom} steps {steps} chunk {cxk}"
+            assert_same(lo, lref, what)
+            assert_same(to, tref, what + " tmp")
+            assert np.array_equal(po[2:400], pref[2:400]), what
+
+
+def test_panel6_long_run_and_switches(orc):
+    """200 steps of panel6 mixed with the other sweeps on a lattice whose
+    last band of interior rows is partial, against the oracle; and a lattice
+    too small for six levels (it runs as panel4 / panel) on the way."""
+    ch = OChannel.geometry(700, 333, "B")
+    _, lref, tref, _ = orc.run(ch, 200)
+    with L.Lattice(dev_channel(ch)) as lat:
+        lat.init_equilibrium()
+        for n, v in ((37, "panel6"), (3, "combine"), (80, "panel6"), (2, "panel4"), (78, "panel6")):
+            lat.step(n, v)
+        lo, to, _ = lat.download()
+    assert_same(lo, lref, "panel6 mixed")
+    assert_same(to, tref, "panel6 mixed tmp")
+    small = OChannel.geometry(90, 70, "A")
+    _, lref, tref, _ = orc.run(small, 15)
+    with L.Lattice(dev_channel(small)) as lat:
+        lat.init_equilibrium()
+        lat.step(15, "panel6")
+        lo, to, _ = lat.download()
+    assert_same(lo, lref, "panel6 small lattice")
+    assert_same(to, tref, "panel6 small lattice tmp")
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_panel6_reduced_equals_single_steps(storage):
+    """In the reduced formats six steps per launch give the bits of six
+    single steps (FP32: the same stochastic rounding per collision index)."""
+    ch = L.Channel(600, 400, 120, 200, 40)
+    with L.Lattice(ch, storage=storage) as a, L.Lattice(ch, storage=storage) as b:
+        a.init_equilibrium(); b.init_equilibrium()
+        a.step(43, "combine"); b.step(43, "panel6")
+        la, ta, pa = a.download(); lb, tb, pb = b.download()
+    assert np.array_equal(la, lb) and np.array_equal(ta, tb) and np.array_equal(pa, pb)
+
+
 @pytest.mark.parametrize("variant,tiles", [
     ("panel", [(0, 0), (3, 5), (7, 254), (130, 17), (4, 1)]),
     ("tight_block", [(0, 0), (3, 30), (5, 62), (20, 126), (50, 62)]),
