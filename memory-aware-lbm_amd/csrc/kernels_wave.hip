@@ -269,16 +269,17 @@ template <int N, typename F> __device__ __forceinline__ void static_for(F &&f) {
 // memory together as one contiguous run per population.
 constexpr int WPG = 8;
 
-// Interior waves, LEVELS time steps per launch (2, or 4 for the deeper
-// sweep).  A wave's 64 lanes are rows Y0-H .. Y0+63-H with H = LEVELS-1 halo
+// Interior waves, LEVELS time steps per launch (2, or 4 / 6 for the deeper
+// sweeps).  A wave's 64 lanes are rows Y0-H .. Y0+63-H with H = LEVELS-1 halo
 // rows on each side: level l (1-based) is valid on lanes l-1 .. 64-l, so the
 // wave owns the 64-2H rows of the last level.  Iteration x computes level 1
 // of column x from the pulled populations and level l of column x-l+1 from
 // level l-1's window of three columns (two kept in registers per level, the
 // third just computed), then stores level LEVELS of column x-H.
 // DEPTH = columns whose loads are in flight (1 or 2); cx % DEPTH == 0.
+// dstep: index of the launch's first collision relative to the counter.
 template <typename S, int DEPTH, int LEVELS>
-__global__ __launch_bounds__(64 * WPG) void k_wpanel_in(Problem p, WaveGrid g,
+__global__ __launch_bounds__(64 * WPG) void k_wpanel_in(Problem p, WaveGrid g, uint32_t dstep,
                                                         const S *__restrict__ P,
                                                         S *__restrict__ Q) {
     using T = typename Calc<S>::T;
@@ -297,7 +298,7 @@ __global__ __launch_bounds__(64 * WPG) void k_wpanel_in(Problem p, WaveGrid g,
     // a partial last band: lanes past the interior's halo read row ly (in
     // bounds; their values reach no owned row)
     const int yl = y < L.ly ? y : L.ly;
-    const uint32_t m = sizeof(S) == 4 ? *p.step : 0u;   // index of the first collision
+    const uint32_t m = sizeof(S) == 4 ? *p.step + dstep : 0u;   // first collision's index
     const uint32_t colb = (uint32_t)L.pitch * (uint32_t)sizeof(S);
     const uint32_t lane_off = (lane >= H && lane < H + ROWS && y < g.r1)
                                   ? (uint32_t)(y + L.yoff) * (uint32_t)sizeof(S) : DROP;
@@ -494,10 +495,10 @@ static WaveGrid wave_grid_deep(const Layout &L, int cx, int storage, int levels)
 bool wpanel_deep_possible(const Layout &L, int cx, int storage, int levels) {
     if (levels != 4 && levels != 6) return false;
     WaveGrid g = wave_grid_deep(L, cx, storage, levels);
-    // the frames of the perimeter launches, widened by up to LEVELS/2 - 1
-    // bands / chunks, must leave an interior
-    const int w = levels / 2 - 1;
-    return g.b_hi - g.b_lo > 2 * w && g.c_hi - g.c_lo > 2 * w;
+    // an interior of at least two bands and two chunks (it then holds the
+    // rings of the perimeter passes, at most LEVELS - 2 nodes thick, many
+    // times over)
+    return g.b_hi - g.b_lo >= 2 && g.c_hi - g.c_lo >= 2;
 }
 
 // Perimeter waves on `side`, interior waves on `st`, joined back into `st`:
@@ -518,8 +519,8 @@ hipError_t launch_wpanel(int storage, const Problem &p, const WaveGrid &g, const
         if (!g.nin) return;
         const S *src = (const S *)P;
         S *dst = (S *)Q;
-        if (depth == 2) k_wpanel_in<S, 2, 2><<<blocks, 64 * WPG, 0, st>>>(p, g, src, dst);
-        else k_wpanel_in<S, 1, 2><<<blocks, 64 * WPG, 0, st>>>(p, g, src, dst);
+        if (depth == 2) k_wpanel_in<S, 2, 2><<<blocks, 64 * WPG, 0, st>>>(p, g, 0u, src, dst);
+        else k_wpanel_in<S, 1, 2><<<blocks, 64 * WPG, 0, st>>>(p, g, 0u, src, dst);
     };
     if (storage == 0) go(double{});
     else if (storage == 1) go(float{});
@@ -530,39 +531,107 @@ hipError_t launch_wpanel(int storage, const Problem &p, const WaveGrid &g, const
     return e;
 }
 
+// ---- rings of a few nodes inside the deep interior ---------------------------
+// The perimeter passes of a deep launch run on the frame around the deep
+// interior rectangle; each pass but the last must also leave its output on a
+// ring of t nodes just inside the rectangle, where the next pass's perimeter
+// waves (their halo lanes and columns compute a first step one node in, from
+// pulls two nodes in) and the next pass's ring pull from.  These few rows and columns are interior
+// nodes (the rectangle's halo stays off the walls); one thread per ring node
+// recomputes the first step of its 3 x 3 neighbourhood from the pass's input
+// and then the second step of the node, the same operations as every other
+// kernel, so the same bits.
+struct Ring {
+    int xa, xb, ya, yb;   // the deep interior rectangle [xa, xb) x [ya, yb)
+    int t;                // ring thickness in nodes
+    int n;                // nodes in the ring
+};
+
+template <typename S>
+__global__ __launch_bounds__(256) void k_ring2(Problem p, Ring r, uint32_t dstep,
+                                               const S *__restrict__ P, S *__restrict__ Q) {
+    using T = typename Calc<S>::T;
+    const Layout &L = p.lay;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= r.n) return;
+    const int w = r.xb - r.xa, band = r.t * w;
+    int x, y;
+    if (k < band) {
+        y = r.ya + k / w;
+        x = r.xa + k % w;
+    } else if (k < 2 * band) {
+        y = r.yb - 1 - (k - band) / w;
+        x = r.xa + (k - band) % w;
+    } else {
+        const int j = k - 2 * band, c = j % (2 * r.t);
+        y = r.ya + r.t + j / (2 * r.t);
+        x = c < r.t ? r.xa + c : r.xb - 1 - (c - r.t);
+    }
+    const uint32_t m = sizeof(S) == 4 ? *p.step + dstep : 0u;
+    if constexpr (sizeof(S) == 4) fp32_round_toward_zero();   // no integer division below
+    T h[9];
+    for_pops([&]<int I>() {
+        // first step of the neighbour this population streams from
+        const int nx = x - cx(I), ny = y - cy(I);
+        const int64_t n = L.node(nx, ny);
+        T g[9];
+        for_pops([&]<int J>() {
+            g[J] = as_calc<S>(P[J * L.plane + n - cx(J) * L.pitch - cy(J)]);
+        });
+        collide_any<S>(p, -1, nx, ny, g, 0., m);
+        h[I] = stored<S>(g[I]);
+    });
+    collide_any<S>(p, -1, x, y, h, 0., m + 1);
+    const int64_t n = L.node(x, y);
+    for_pops([&]<int I>() { Q[I * L.plane + n] = to_storage<S>(h[I]); });
+}
+
 hipError_t launch_wpanel_deep(int storage, const Problem &p, int cx, int levels, const void *P,
                               void *const *mid, void *Q, hipStream_t st, hipStream_t side,
                               hipEvent_t fork, hipEvent_t join) {
     if (!wpanel_deep_possible(p.lay, cx, storage, levels)) return hipErrorInvalidValue;
-    const WaveGrid g = wave_grid_deep(p.lay, cx, storage, levels);
+    const Layout &L = p.lay;
+    const WaveGrid g = wave_grid_deep(L, cx, storage, levels);
     const int nmid = levels / 2 - 1;
     const int depth = g.cx % 2 == 0 ? 2 : 1;
-    const int blocks = (g.nin + WPG - 1) / WPG;
+    Ring ring;
+    ring.xa = L.lx + 1 - (g.ncx - g.c_lo) * g.cx;
+    ring.xb = L.lx + 1 - (g.ncx - g.c_hi) * g.cx;
+    ring.ya = g.r0;
+    ring.yb = g.r1;
     hipError_t e = hipEventRecord(fork, st);
     if (e == hipSuccess) e = hipStreamWaitEvent(side, fork, 0);
     if (e != hipSuccess) return e;
     auto go = [&](auto tag) {
         using T = decltype(tag);
-        // perimeter launch j advances the frame by steps 2j+1, 2j+2, from P or
-        // mid[j-1] into mid[j] or Q, over the frame widened by nmid - j bands /
-        // chunks, so that its output covers every value launch j+1 pulls
+        // perimeter pass j advances the frame by steps 2j+1, 2j+2, from P or
+        // mid[j-1] into mid[j] or Q; all but the last also fill a ring inside
+        // the interior rectangle: 2 nodes deep for the pass before the last
+        // (the perimeter waves' halo computes a first step one node in, from
+        // pulls two nodes in), and 2 deeper for each pass before that (a
+        // ring node t-1 deep takes its neighbour's first step t deep, from
+        // pulls t+1 deep)
         for (int j = 0; j <= nmid; ++j) {
-            WaveGrid gj = g;
-            const int w = nmid - j;
-            gj.b_lo += w; gj.b_hi -= w; gj.c_lo += w; gj.c_hi -= w;
-            gj.nedge = gj.ntasks - (gj.b_hi - gj.b_lo) * (gj.c_hi - gj.c_lo);
             const T *src = (const T *)(j == 0 ? P : mid[j - 1]);
             T *dst = (T *)(j == nmid ? Q : mid[j]);
-            k_wpanel_edge<T><<<gj.nedge, 64, 0, side>>>(p, gj, 2u * j, src, dst);
+            k_wpanel_edge<T><<<g.nedge, 64, 0, side>>>(p, g, 2u * j, src, dst);
+            if (j < nmid) {
+                Ring r = ring;
+                r.t = 2 + 2 * (nmid - 1 - j);
+                const int wx = r.xb - r.xa, wy = r.yb - r.ya;
+                r.n = 2 * r.t * wx + 2 * r.t * (wy - 2 * r.t);
+                k_ring2<T><<<(r.n + 255) / 256, 256, 0, side>>>(p, r, 2u * j, src, dst);
+            }
         }
+        const int blocks = (g.nin + WPG - 1) / WPG;
         const T *src = (const T *)P;
         T *dst = (T *)Q;
         if (levels == 4) {
-            if (depth == 2) k_wpanel_in<T, 2, 4><<<blocks, 64 * WPG, 0, st>>>(p, g, src, dst);
-            else k_wpanel_in<T, 1, 4><<<blocks, 64 * WPG, 0, st>>>(p, g, src, dst);
+            if (depth == 2) k_wpanel_in<T, 2, 4><<<blocks, 64 * WPG, 0, st>>>(p, g, 0u, src, dst);
+            else k_wpanel_in<T, 1, 4><<<blocks, 64 * WPG, 0, st>>>(p, g, 0u, src, dst);
         } else {
-            if (depth == 2) k_wpanel_in<T, 2, 6><<<blocks, 64 * WPG, 0, st>>>(p, g, src, dst);
-            else k_wpanel_in<T, 1, 6><<<blocks, 64 * WPG, 0, st>>>(p, g, src, dst);
+            if (depth == 2) k_wpanel_in<T, 2, 6><<<blocks, 64 * WPG, 0, st>>>(p, g, 0u, src, dst);
+            else k_wpanel_in<T, 1, 6><<<blocks, 64 * WPG, 0, st>>>(p, g, 0u, src, dst);
         }
     };
     if (storage == 0) go(double{});

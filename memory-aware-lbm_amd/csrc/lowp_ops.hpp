@@ -1,9 +1,12 @@
 // lowp_ops.hpp -- node operators on stored deviations g_i = f_i - w_i (the
 // reduced-precision storage forms).  FP32 storage: the collision runs in FP64
 // and each result is rounded to FP32 stochastically.  FP16 storage: the
-// collision runs in FP32 and results are rounded to nearest.  Fused
-// multiply-add is allowed here whatever the translation unit's
-// -ffp-contract, so every kernel that includes this header computes the same
+// collision runs in FP32 and results are rounded to nearest.  Every fused
+// multiply-add is written out (fma / fmaf) and the compiler forms no others
+// (contraction off): which product a contracting compiler fuses depends on
+// how often each product is used, i.e. on the code around the collision, and
+// a kernel that needs one population of a collision would round differently
+// from one that needs all nine.  Written out, every kernel computes the same
 // bits.
 #pragma once
 
@@ -19,6 +22,18 @@ namespace lbm {
 template <typename S> __device__ __forceinline__ S narrow(float v) {
     if constexpr (sizeof(S) == 2) __asm__ volatile("" : "+v"(v));
     return (S)v;
+}
+
+template <int I> __device__ __forceinline__ double cdotu_d(double ux, double uy) {
+    if constexpr (I == 0) return 0.;
+    else if constexpr (I == 1) return ux;
+    else if constexpr (I == 2) return uy;
+    else if constexpr (I == 3) return -ux;
+    else if constexpr (I == 4) return -uy;
+    else if constexpr (I == 5) return ux + uy;
+    else if constexpr (I == 6) return uy - ux;
+    else if constexpr (I == 7) return -ux - uy;
+    else return ux - uy;
 }
 
 template <int I> __device__ __forceinline__ float cdotu_f(float ux, float uy) {
@@ -38,7 +53,7 @@ template <int I> __device__ __forceinline__ float cdotu_f(float ux, float uy) {
 // j - j drho / (1 + drho) and rho X as X + drho X, so the rounding of 1 + drho
 // only touches terms that are already O(drho) small.
 __device__ __forceinline__ void bgk_dev(float g[9], float omega) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(off)
     float north = g[2] + g[5] + g[6];
     float mid = g[0] + g[1] + g[3];
     float south = g[4] + g[7] + g[8];
@@ -46,17 +61,17 @@ __device__ __forceinline__ void bgk_dev(float g[9], float omega) {
     float jx = (g[1] + g[5] + g[8]) - (g[3] + g[6] + g[7]);
     float jy = north - south;
     float shrink = drho / (1.f + drho);
-    float ux = jx - jx * shrink, uy = jy - jy * shrink;
-    float usq = ux * ux + uy * uy;
+    float ux = fmaf(-jx, shrink, jx), uy = fmaf(-jy, shrink, jy);
+    float usq = fmaf(ux, ux, uy * uy);
+    float u15 = -1.5f * usq;
     // moving populations first; the rest population's equilibrium closes the
     // mass balance exactly (sum geq = drho), so FP32 rounding of the
     // equilibrium cannot bias the density step after step
     float rest = drho;
     for_pops([&]<int I>() {
         if constexpr (I != 0) {
-#pragma clang fp contract(fast)
             float cu = cdotu_f<I>(ux, uy);
-            float X = 3.f * cu + 4.5f * cu * cu - 1.5f * usq;
+            float X = fmaf(4.5f * cu, cu, fmaf(3.f, cu, u15));
             // the weights 1/9 and 1/36 are not FP32 numbers: a one-term
             // product would scale every moving equilibrium by the same
             // (1 + 7.45e-9) and push that much extra momentum into the flow
@@ -64,13 +79,13 @@ __device__ __forceinline__ void bgk_dev(float g[9], float omega) {
             // two-term weight leaves only unbiased rounding
             constexpr float w_hi = (float)wgt(I);
             constexpr float w_lo = (float)(wgt(I) - (double)w_hi);
-            float Y = drho + (X + drho * X);
-            float geq = Y * w_hi + Y * w_lo;
+            float Y = drho + fmaf(drho, X, X);
+            float geq = fmaf(Y, w_hi, Y * w_lo);
             rest -= geq;
-            g[I] += omega * (geq - g[I]);
+            g[I] = fmaf(omega, geq - g[I], g[I]);
         }
     });
-    g[0] += omega * (rest - g[0]);
+    g[0] = fmaf(omega, rest - g[0], g[0]);
 }
 
 // ---- FP32 storage: FP64 collision, stochastic rounding --------------------------
@@ -136,7 +151,7 @@ __device__ __forceinline__ void fp32_round_toward_zero() {
 // bgk_dev; the mass-closing rest population is kept), rounded stochastically.
 template <bool RZ>
 __device__ __forceinline__ void bgk_f32s(float g[9], double omega, SrSeed h) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(off)
     double d[9];
     for_pops([&]<int I>() { d[I] = g[I]; });
     const double north = d[2] + d[5] + d[6], mid = d[0] + d[1] + d[3], south = d[4] + d[7] + d[8];
@@ -148,19 +163,19 @@ __device__ __forceinline__ void bgk_f32s(float g[9], double omega, SrSeed h) {
     inv = fma(inv, fma(-a, inv, 1.0), inv);
     const double ux = ((d[1] + d[5] + d[8]) - (d[3] + d[6] + d[7])) * inv;
     const double uy = (north - south) * inv;
-    const double usq = ux * ux + uy * uy;
+    const double usq = fma(ux, ux, uy * uy);
+    const double u15 = -1.5 * usq;
     double rest = drho;
     for_pops([&]<int I>() {
         if constexpr (I != 0) {
-#pragma clang fp contract(fast)
-            const double cu = cx(I) * ux + cy(I) * uy;
-            const double X = 3.0 * cu + 4.5 * cu * cu - 1.5 * usq;
-            const double geq = wgt(I) * (drho + (X + drho * X));
+            const double cu = cdotu_d<I>(ux, uy);
+            const double X = fma(4.5 * cu, cu, fma(3.0, cu, u15));
+            const double geq = wgt(I) * (drho + fma(drho, X, X));
             rest -= geq;
-            g[I] = sr_f32<I, RZ>(d[I] + omega * (geq - d[I]), h);
+            g[I] = sr_f32<I, RZ>(fma(omega, geq - d[I], d[I]), h);
         }
     });
-    g[0] = sr_f32<0, RZ>(d[0] + omega * (rest - d[0]), h);
+    g[0] = sr_f32<0, RZ>(fma(omega, rest - d[0], d[0]), h);
 }
 
 // a wall node's FP64 result back to the storage format
