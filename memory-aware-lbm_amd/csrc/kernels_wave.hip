@@ -1,8 +1,8 @@
 // kernels_wave.hip -- several time steps per launch with the intermediate
 // steps held in registers of one wave64: the panel sweep
 // (collide_tight_panel_iy, lb.c:1402-1538, two steps) mapped onto CDNA
-// wavefronts, and the same march four steps deep (no counterpart in the
-// reference).
+// wavefronts, and the same march four or six steps deep (no counterpart in
+// the reference).
 //
 // A wave owns a band of rows and a chunk of columns, and marches x.  For two
 // steps its 64 lanes are the band's 62 rows plus one halo row on each side
@@ -14,7 +14,10 @@
 // populations; x-2: populations 1, 5, 8, the cx = +1 sources) and takes its
 // neighbours' with one lane shift up and one down (6 values per column).
 // Nothing goes through LDS.  Four steps stack three such levels (58 owned
-// rows, three halo rows each side).
+// rows, three halo rows each side), six steps five (54 owned rows).  The
+// deep sweeps' perimeter frame advances two steps per pass through further
+// buffers, each pass but the last also filling a ring a few nodes deep just
+// inside the interior rectangle for the next pass (k_ring2).
 //
 // Waves whose rows or columns touch the channel perimeter run a separate,
 // general kernel (k_wpanel_edge: wall operators, outlet extrapolation,

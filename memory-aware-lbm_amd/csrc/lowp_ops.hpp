@@ -95,7 +95,7 @@ __device__ __forceinline__ void bgk_dev(float g[9], float omega) {
 // of a big channel those errors are coherent and act as a steady forcing.
 // On 16384 x 4096 after 20000 steps (Re 100) the velocity error against FP64
 // is 3.0e-5 with FP32 collisions, 2.9e-5 with FP64 collisions rounded to
-// nearest, 2.6e-5 with FP32 collisions rounded stochastically, and 5.4e-8
+// nearest, 2.6e-5 with FP32 collisions rounded stochastically, and 5.2e-8
 // with FP64 collisions (walls included) rounded stochastically.  The random
 // bits are a hash of (node, collision index, population), so a run is
 // deterministic and every kernel rounds a given node update identically.
