@@ -557,18 +557,20 @@ __global__ __launch_bounds__(256) void k_ring2(Problem p, Ring r, uint32_t dstep
     const Layout &L = p.lay;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= r.n) return;
-    const int w = r.xb - r.xa, band = r.t * w;
+    // y runs fastest (neighbouring threads read neighbouring elements): the
+    // bottom and top rows column by column, then the left and right columns
+    const int w = r.xb - r.xa, band = r.t * w, hs = r.yb - r.ya - 2 * r.t;
     int x, y;
     if (k < band) {
-        y = r.ya + k / w;
-        x = r.xa + k % w;
+        x = r.xa + k / r.t;
+        y = r.ya + k % r.t;
     } else if (k < 2 * band) {
-        y = r.yb - 1 - (k - band) / w;
-        x = r.xa + (k - band) % w;
+        x = r.xa + (k - band) / r.t;
+        y = r.yb - r.t + (k - band) % r.t;
     } else {
-        const int j = k - 2 * band, c = j % (2 * r.t);
-        y = r.ya + r.t + j / (2 * r.t);
-        x = c < r.t ? r.xa + c : r.xb - 1 - (c - r.t);
+        const int j = k - 2 * band, c = j / hs;
+        y = r.ya + r.t + j % hs;
+        x = c < r.t ? r.xa + c : r.xb - 2 * r.t + c;
     }
     const uint32_t m = sizeof(S) == 4 ? *p.step + dstep : 0u;
     if constexpr (sizeof(S) == 4) fp32_round_toward_zero();   // no integer division below
