@@ -539,11 +539,11 @@ hipError_t launch_wpanel(int storage, const Problem &p, const WaveGrid &g, const
 // interior rectangle; each pass but the last must also leave its output on a
 // ring of t nodes just inside the rectangle, where the next pass's perimeter
 // waves (their halo lanes and columns compute a first step one node in, from
-// pulls two nodes in) and the next pass's ring pull from.  These few rows and columns are interior
-// nodes (the rectangle's halo stays off the walls); one thread per ring node
-// recomputes the first step of its 3 x 3 neighbourhood from the pass's input
-// and then the second step of the node, the same operations as every other
-// kernel, so the same bits.
+// pulls two nodes in) and the next pass's ring pull from.  These few rows
+// and columns are interior nodes (the rectangle's halo stays off the walls);
+// one thread per ring node recomputes the first step of its 3 x 3
+// neighbourhood from the pass's input and then the second step of the node,
+// the same operations as every other kernel, so the same bits.
 struct Ring {
     int xa, xb, ya, yb;   // the deep interior rectangle [xa, xb) x [ya, yb)
     int t;                // ring thickness in nodes
