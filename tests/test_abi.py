@@ -89,3 +89,48 @@ def test_driver_usage_message():
     exe = os.path.join(PKG, "unsteady_gpu")
     r = subprocess.run([exe], capture_output=True, text=True)
     assert r.returncode == 2 and "usage" in r.stderr
+
+
+def _valid_problem():
+    import lbm_amd
+    p = lbm_amd._Problem()
+    p.lx, p.ly, p.obst_x, p.obst_y, p.obst_r = 64, 32, 12, 16, 3
+    p.omega = 1.5
+    for s in range(4):
+        p.side[s].kind, p.side[s].family = lbm_amd.BC_VELOCITY, lbm_amd.BC_REGULARIZED
+    p.side[lbm_amd.RIGHT].kind = lbm_amd.BC_PRESSURE
+    p.zgb, p.storage = 1, lbm_amd.STORE_F64
+    return p
+
+
+@pytest.mark.parametrize("field,value,message", [
+    ("lx", 2, "lx and ly must be >= 3"),
+    ("ly", 0, "lx and ly must be >= 3"),
+    ("lx", 65535, "lattice too large"),          # (lx+2)(ly+2) > 2^31 with ly = 32766
+    ("omega", 0.0, "omega must lie in (0, 2]"),
+    ("omega", 2.5, "omega must lie in (0, 2]"),
+    ("storage", 7, "unknown storage format"),
+    ("side_kind", 9, "side kind must be velocity or pressure"),
+    ("zgb_left_only", 1, "zgb needs a pressure operator on the right side"),
+])
+def test_create_rejects_bad_problems_without_gpu(field, value, message):
+    """lbm_hip_create validates the problem before any device call: the
+    degenerate and oversized lattices, and the other invalid settings, come
+    back as LBM_ERR_ARG with a message and no context."""
+    import ctypes as C
+    import lbm_amd
+    lib = lbm_amd.library()
+    p = _valid_problem()
+    if field == "lx" and value == 65535:
+        p.lx, p.ly = 65535, 32766
+    elif field == "side_kind":
+        p.side[0].kind = value
+    elif field == "zgb_left_only":
+        p.side[lbm_amd.RIGHT].kind = lbm_amd.BC_VELOCITY
+    else:
+        setattr(p, field, value)
+    h = C.c_void_p(1234)
+    rc = lib.lbm_hip_create(C.byref(p), C.byref(h))
+    assert rc == -1                                        # LBM_ERR_ARG
+    assert h.value is None
+    assert message in lib.lbm_hip_last_error().decode()

@@ -263,6 +263,28 @@ def test_panel6_reduced_equals_single_steps(storage):
     assert np.array_equal(la, lb) and np.array_equal(ta, tb) and np.array_equal(pa, pb)
 
 
+@pytest.mark.parametrize("lx,ly", [(3, 3), (3, 8), (4, 3), (5, 4), (9, 3)])
+def test_smallest_lattices(orc, lx, ly):
+    """The smallest channels the API accepts (3 x 3 and up), a disc on the
+    corner included: every sweep bit-identical to the oracle, except that the
+    two-step and deeper sweeps refuse lx < 4 with an error, as documented
+    (the deeper sweeps fall back to the two-step one on lattices this small)."""
+    ch = OChannel.geometry(lx, ly, "A")
+    _, lref, tref, pref = orc.run(ch, 7)
+    for v in ("origin", "combine", "tight", "tight_block", "panel", "panel4", "panel6"):
+        with L.Lattice(dev_channel(ch)) as lat:
+            lat.init_equilibrium()
+            if lx < 4 and v in ("tight_block", "panel", "panel4", "panel6"):
+                with pytest.raises(L.LbmError):
+                    lat.step(7, v)
+                continue
+            lat.step(7, v)
+            lo, to, po = lat.download()
+        assert_same(lo, lref, f"{v} {lx}x{ly}")
+        assert_same(to, tref, f"{v} {lx}x{ly} tmp")
+        assert np.array_equal(po[2:ly], pref[2:ly])
+
+
 @pytest.mark.parametrize("variant,tiles", [
     ("panel", [(0, 0), (3, 5), (7, 254), (130, 17), (4, 1)]),
     ("tight_block", [(0, 0), (3, 30), (5, 62), (20, 126), (50, 62)]),
