@@ -115,8 +115,9 @@ def measured_traffic(variant):
 
 
 def batching(args):
-    """The batching workload: seeded uneven channels, one persistent launch
-    per step, hipGraph replay, autotuned launch shape; FP64 and FP32."""
+    """The batching workload: seeded uneven channels, hipGraph replay,
+    autotuned launch shape (FP64: two steps per launch over LDS tiles, or one
+    persistent launch per step; FP32: one per step); FP64 and FP32."""
     from lbm_amd.batch import Batch, autotune, channels_from_seed
     chans = channels_from_seed(args.batch, seed=args.batch_seed)
     out = {"channels": len(chans), "seed": args.batch_seed,
@@ -133,7 +134,8 @@ def batching(args):
             out[storage] = {"glups": round(glups, 4), "ms_per_step": round(ms / args.batch_steps, 5),
                             "achieved_gbs": round(glups * bpu, 1),
                             "frac": round(glups * bpu / HBM_PEAK_GBS, 4),
-                            "launch": {"ty_cap": tune["ty_cap"], "wgs_per_cu": tune["wgs_per_cu"],
+                            "launch": {"depth": tune["depth"], "tile_cols": tune["tile_cols"],
+                                       "ty_cap": tune["ty_cap"], "wgs_per_cu": tune["wgs_per_cu"],
                                        "graph_steps": tune["graph_steps"]},
                             "footprint_gb": round(b.footprint() / 1e9, 3)}
     return out

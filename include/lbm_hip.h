@@ -235,7 +235,9 @@ int lbm_hip_copy_bandwidth(size_t bytes, int iters, double *ms_per_copy);
  * channels must use the same storage format.  A step of the batch is one
  * persistent launch over a table of 256-node tiles of every channel; each
  * node runs the single-lattice node step, so every channel evolves bit for
- * bit as it would in its own context (fused pull variant). */
+ * bit as it would in its own context (fused pull variant).  FP64 batches
+ * advance two steps per launch by default (lbm_hip_batch_set_depth), with
+ * the LDS tiles of the tight_block sweep: the same bits. */
 typedef struct lbm_batch lbm_batch;
 
 int lbm_hip_batch_create(const lbm_problem *probs, int n, lbm_batch **out);
@@ -247,6 +249,12 @@ int lbm_hip_batch_init_equilibrium(lbm_batch *b, const double *const *init_ux);
  * captured per hipGraph (0 = launch eagerly).  0 leaves a knob unchanged
  * except graph_steps. */
 int lbm_hip_batch_set_launch(lbm_batch *b, int wgs_per_cu, int ty_cap, int graph_steps);
+/* Steps per launch: 1 (the persistent one-step launch above) or 2 (one
+ * workgroup per tile of tile_cols columns x 62 rows of every channel, the
+ * first step in LDS; FP64 only and every channel's lx >= 4,
+ * LBM_ERR_UNSUPPORTED otherwise; the default for such batches).  tile_cols in [3, 33] (default 8), 0 keeps it.  Odd
+ * remainders and the first step after init_equilibrium run one step. */
+int lbm_hip_batch_set_depth(lbm_batch *b, int depth, int tile_cols);
 int lbm_hip_batch_step(lbm_batch *b, int nsteps);
 int lbm_hip_batch_sync(lbm_batch *b);
 int lbm_hip_batch_time(lbm_batch *b, int nsteps, double *ms);

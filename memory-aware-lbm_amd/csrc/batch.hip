@@ -12,6 +12,12 @@
 // (node_step.hpp), so every channel's result is bit-identical to running it
 // alone.  K consecutive steps can be captured into a hipGraph and replayed.
 //
+// FP64 batches can instead advance two steps per launch (depth 2): one
+// workgroup per two-step LDS tile of the single-lattice tight_block sweep
+// (block_tile.hpp), the tiles of all channels in one flat table, renumbered
+// so each XCD takes a contiguous range.  The same tile body, so the same
+// bits; the lattice crosses HBM once per two steps.
+//
 // No counterpart in the reference (one lattice per process there).
 #include <hip/hip_runtime.h>
 
@@ -21,6 +27,7 @@
 
 #include "../../include/lbm_hip.h"
 #include "host_util.hpp"
+#include "block_tile.hpp"
 #include "launch.hpp"
 #include "node_step.hpp"
 
@@ -31,7 +38,17 @@ struct BatchEntry {
     int64_t off;    // element offset of the channel in each pool
     int64_t item0;  // first work item of the channel
     int ty, tx, nty;
+    // two-step tiles: btx columns x (BATCH_TYP - 2) rows, bnbx x bnby of
+    // them, numbered from tile0 (column strip major)
+    int64_t tile0;
+    int btx, bnbx, bnby;
 };
+
+constexpr int BATCH_TYP = 64;  // two-step tile rows + the one-node ring
+
+inline size_t batch_block_lds(int tile_cols) {
+    return 9 * (size_t)(tile_cols + 2) * BATCH_TYP * sizeof(double);
+}
 
 template <typename S, bool PULL>
 __global__ __launch_bounds__(256) void k_batch(const BatchEntry *__restrict__ ents,
@@ -56,6 +73,21 @@ __global__ __launch_bounds__(256) void k_batch(const BatchEntry *__restrict__ en
         else
             node_step_lp<S, PULL, false>(e.p, x, y, A + e.off, B + e.off, step);
     }
+}
+
+// Two steps of every channel, one workgroup per tile (FP64, collided state
+// in A, collided state two steps later in B).
+__global__ __launch_bounds__(BLOCK_THREADS) void k_batch_block(const BatchEntry *__restrict__ ents,
+                                                             const int *__restrict__ tile_chan,
+                                                             const double *__restrict__ A,
+                                                             double *__restrict__ B) {
+    extern __shared__ double lds[];  // [9][btx+2][BATCH_TYP]
+    const int t = xcd_block(blockIdx.x, gridDim.x);
+    const int c = __builtin_amdgcn_readfirstlane(tile_chan[t]);
+    const BatchEntry &e = ents[c];
+    const int local = t - (int)e.tile0;
+    const int by = local % e.bnby, bx = local / e.bnby;
+    block_tile<BATCH_TYP>(e.p, e.btx, e.bnbx, bx, by, A + e.off, B + e.off, lds);
 }
 
 template <typename S>
@@ -101,6 +133,12 @@ struct lbm_batch {
     int ty_cap = 256;
     int graph_steps = 0;
     hipGraphExec_t graph[2] = {nullptr, nullptr};  // by starting buffer
+    int graph_flip[2] = {0, 0};                     // buffer swaps per replay, mod 2
+    int depth = 1;                                  // steps per launch (2: FP64 tiles)
+    int tile_cols = 8;                              // columns of a two-step tile
+    int min_lx = 0;                                 // two-step tiles need lx >= 4
+    int *d_tile_chan = nullptr;                     // channel of each two-step tile
+    int64_t ntiles = 0;
     std::vector<int> ly_of, lx_of;
 };
 
@@ -120,10 +158,10 @@ void drop_graphs(lbm_batch *b) {
         }
 }
 
-// tile shapes and the work table for the current ty_cap
+// tile shapes and the work tables for the current ty_cap and tile_cols
 int build_items(lbm_batch *b) {
-    std::vector<int> chan;
-    int64_t item = 0;
+    std::vector<int> chan, tchan;
+    int64_t item = 0, tile = 0;
     for (int c = 0; c < b->n; ++c) {
         BatchEntry &e = b->ents[c];
         int ty = std::min(b->ty_cap, std::max(32, ceil_pow2(e.p.lay.ly)));
@@ -135,12 +173,25 @@ int build_items(lbm_batch *b) {
         int64_t cnt = (int64_t)ntx * e.nty;
         chan.insert(chan.end(), (size_t)cnt, c);
         item += cnt;
+        e.btx = b->tile_cols;
+        e.bnbx = (e.p.lay.lx + e.btx - 1) / e.btx;
+        e.bnby = (e.p.lay.ly + BATCH_TYP - 3) / (BATCH_TYP - 2);
+        e.tile0 = tile;
+        int64_t nt = (int64_t)e.bnbx * e.bnby;
+        tchan.insert(tchan.end(), (size_t)nt, c);
+        tile += nt;
     }
+    if (tile > INT32_MAX) return fail(LBM_ERR_ARG, "batch has more than 2^31 two-step tiles");
     if (b->d_item_chan) (void)hipFree(b->d_item_chan);
+    if (b->d_tile_chan) (void)hipFree(b->d_tile_chan);
     b->d_item_chan = nullptr;
+    b->d_tile_chan = nullptr;
     b->nitems = item;
+    b->ntiles = tile;
     HIPCHK(hipMalloc((void **)&b->d_item_chan, chan.size() * sizeof(int)));
     HIPCHK(hipMemcpy(b->d_item_chan, chan.data(), chan.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc((void **)&b->d_tile_chan, tchan.size() * sizeof(int)));
+    HIPCHK(hipMemcpy(b->d_tile_chan, tchan.data(), tchan.size() * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(b->d_ents, b->ents.data(), b->ents.size() * sizeof(BatchEntry),
                      hipMemcpyHostToDevice));
     drop_graphs(b);
@@ -168,12 +219,26 @@ hipError_t launch_step(lbm_batch *b, bool pull) {
     return e == hipSuccess ? launch_bump(b->d_step, 1, b->stream) : e;
 }
 
+// two steps of every channel from the collided state (FP64 only)
+hipError_t launch_pair(lbm_batch *b) {
+    k_batch_block<<<(unsigned)b->ntiles, BLOCK_THREADS, batch_block_lds(b->tile_cols), b->stream>>>(
+        b->d_ents, b->d_tile_chan, (const double *)b->pool[b->cur], (double *)b->pool[1 - b->cur]);
+    return hipGetLastError();
+}
+
+// steps of the next launch: two where the depth and the state allow it
+int launch_depth(const lbm_batch *b, int left, bool collided) {
+    return b->depth == 2 && collided && left >= 2 ? 2 : 1;
+}
+
 int eager_steps(lbm_batch *b, int nsteps) {
-    for (int k = 0; k < nsteps; ++k) {
-        HIPCHK(launch_step(b, b->collided != 0));
+    while (nsteps > 0) {
+        const int k = launch_depth(b, nsteps, b->collided != 0);
+        HIPCHK(k == 2 ? launch_pair(b) : launch_step(b, b->collided != 0));
         b->cur = 1 - b->cur;
         b->collided = 1;
-        b->steps += 1;
+        b->steps += k;
+        nsteps -= k;
     }
     return LBM_OK;
 }
@@ -185,12 +250,17 @@ int capture(lbm_batch *b, int start) {
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
     hipError_t e = hipSuccess;
-    for (int k = 0; k < b->graph_steps && e == hipSuccess; ++k) {
-        e = launch_step(b, true);
+    int flips = 0;
+    for (int left = b->graph_steps; left > 0 && e == hipSuccess;) {
+        const int k = launch_depth(b, left, true);
+        e = k == 2 ? launch_pair(b) : launch_step(b, true);
         b->cur = 1 - b->cur;
+        ++flips;
+        left -= k;
     }
     hipError_t e2 = hipStreamEndCapture(b->stream, &g);
     b->cur = saved_cur;
+    b->graph_flip[start] = flips & 1;
     if (e != hipSuccess) { if (g) (void)hipGraphDestroy(g); return hip_fail(e, "capture launch"); }
     if (e2 != hipSuccess) return hip_fail(e2, "hipStreamEndCapture");
     hipError_t e3 = hipGraphInstantiate(&b->graph[start], g, nullptr, nullptr, 0);
@@ -213,7 +283,7 @@ int step_impl(lbm_batch *b, int nsteps) {
                 if (rc) return rc;
             }
             HIPCHK(hipGraphLaunch(b->graph[b->cur], b->stream));
-            if (b->graph_steps & 1) b->cur = 1 - b->cur;
+            if (b->graph_flip[b->cur]) b->cur = 1 - b->cur;
             b->steps += b->graph_steps;
             nsteps -= b->graph_steps;
         }
@@ -245,6 +315,9 @@ int lbm_hip_batch_create(const lbm_problem *probs, int n, lbm_batch **out) {
     if (!b) return fail(LBM_ERR_NOMEM, "host allocation");
     b->n = n;
     b->storage = storage;
+    b->min_lx = probs[0].lx;
+    for (int c = 1; c < n; ++c) b->min_lx = std::min(b->min_lx, probs[c].lx);
+    b->depth = storage == LBM_STORE_F64 && b->min_lx >= 4 ? 2 : 1;
     b->ents.resize(n);
     int dev = 0;
     hipDeviceProp_t prop;
@@ -278,6 +351,9 @@ int lbm_hip_batch_create(const lbm_problem *probs, int n, lbm_batch **out) {
     };
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return bail(e, "hipStreamCreate");
+    e = hipFuncSetAttribute((const void *)k_batch_block,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (e != hipSuccess) return bail(e, "hipFuncSetAttribute");
     for (int k = 0; k < 2; ++k) {
         e = hipMalloc(&b->pool[k], (size_t)off * esize(storage));
         if (e != hipSuccess) return bail(e, "hipMalloc pool");
@@ -332,6 +408,7 @@ int lbm_hip_batch_destroy(lbm_batch *b) {
     if (b->params) (void)hipFree(b->params);
     if (b->d_ents) (void)hipFree(b->d_ents);
     if (b->d_item_chan) (void)hipFree(b->d_item_chan);
+    if (b->d_tile_chan) (void)hipFree(b->d_tile_chan);
     if (b->d_step) (void)hipFree(b->d_step);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
@@ -396,6 +473,27 @@ int lbm_hip_batch_set_launch(lbm_batch *b, int wgs_per_cu, int ty_cap, int graph
         }
     }
     drop_graphs(b);
+    return LBM_OK;
+}
+
+int lbm_hip_batch_set_depth(lbm_batch *b, int depth, int tile_cols) {
+    if (!b || (depth != 1 && depth != 2) || tile_cols < 0)
+        return fail(LBM_ERR_ARG, "bad depth setting");
+    if (tile_cols > 0 && (tile_cols < 3 || batch_block_lds(tile_cols) > 160 * 1024))
+        return fail(LBM_ERR_ARG, "tile_cols must be in [3, 33]");
+    if (depth == 2 && b->storage != LBM_STORE_F64)
+        return fail(LBM_ERR_UNSUPPORTED, "two steps per launch need FP64 storage");
+    // the outlet extrapolation of a tile's second step reads the first step
+    // of column lx-3 (as the single-lattice two-step sweeps)
+    if (depth == 2 && b->min_lx < 4)
+        return fail(LBM_ERR_UNSUPPORTED, "two steps per launch need lx >= 4 in every channel");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    drop_graphs(b);
+    b->depth = depth;
+    if (tile_cols > 0 && tile_cols != b->tile_cols) {
+        b->tile_cols = tile_cols;
+        return build_items(b);
+    }
     return LBM_OK;
 }
 
@@ -504,8 +602,8 @@ int64_t lbm_hip_batch_steps_done(const lbm_batch *b) { return b ? b->steps : -1;
 
 size_t lbm_hip_batch_footprint(const lbm_batch *b) {
     if (!b) return 0;
-    return 2 * (size_t)b->pool_elems * esize(b->storage) + (size_t)b->nitems * sizeof(int) +
-           b->ents.size() * sizeof(BatchEntry);
+    return 2 * (size_t)b->pool_elems * esize(b->storage) +
+           (size_t)(b->nitems + b->ntiles) * sizeof(int) + b->ents.size() * sizeof(BatchEntry);
 }
 
 }  // extern "C"

@@ -7,122 +7,25 @@
 // step's results live in LDS instead of cache:
 //
 //  k_block  a workgroup owns a 2-D tile; the first step covers the tile plus
-//           a one-node ring in LDS, the second step the tile itself.
+//           a one-node ring in LDS, the second step the tile itself
+//           (block_tile.hpp, shared with the batched channels).
 //  (the panel sweep keeps its first step in wave registers instead:
 //   kernels_wave.hip)
-//
-// k_block reads only the collided buffer P (post-collision of step m-1, ghost
-// ring = unknown populations for step m) and write only Q (post-collision of
-// step m+1 and the ghost ring for step m+2), so workgroups never race.  The
-// first step's unknown populations come from P's ghost ring; the second
-// step's are the node's own post-collision populations of step m-1, i.e.
-// P(x) itself.  The outlet density of the second step is extrapolated from
-// the densities of columns lx-2 and lx-1 taken just before their second
-// collision, as the reference captures them (lb.c:968-977, 1025-1029);
-// every launch therefore gives the tile that holds column lx at least three
-// columns (tiles are aligned to the right end).
 #include "launch.hpp"
-#include "collide_f64.hpp"
+#include "block_tile.hpp"
 
 namespace lbm {
 
-// LDS-only workgroup barrier: orders the tile's LDS traffic without waiting
-// for outstanding global loads (a plain __syncthreads() would drain them).
-__device__ __forceinline__ void lds_barrier() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
-__device__ __forceinline__ void pull_node(const Layout &L, const double *__restrict__ P, int x,
-                                          int y, double f[9]) {
-    int64_t n = L.node(x, y);
-    for_pops([&]<int I>() { f[I] = P[I * L.plane + n - cx(I) * L.pitch - cy(I)]; });
-}
-
-__device__ __forceinline__ double pulled_density(const Layout &L, const double *__restrict__ P,
-                                                 int x, int y) {
-    double g[9];
-    pull_node(L, P, x, y, g);
-    return rho_of(g);
-}
-
-// First step of node (x, y): post-stream from P (ghost ring included), then
-// the node's collision with the outlet density of this step.
-__device__ __forceinline__ void first_step(const Problem &p, const double *__restrict__ P, int x,
-                                           int y, double f[9]) {
-    int s = side_of(p, x, y);
-    double rp0 = 0.;
-    if (s == RIGHT && p.zgb)
-        rp0 = zgb_rho(pulled_density(p.lay, P, p.lay.lx - 1, y),
-                      pulled_density(p.lay, P, p.lay.lx - 2, y));
-    collide_class(p, s, x, y, f, rp0);
-}
-
 // ---- 2-D tiles --------------------------------------------------------------
-
-constexpr int BLOCK_THREADS = 256;
 
 template <int TYP>  // padded tile height (rows incl. the one-node ring), power of two
 __global__ __launch_bounds__(BLOCK_THREADS) void k_block(Problem p, BlockGrid g,
                                                          const double *__restrict__ P,
                                                          double *__restrict__ Q) {
-    extern __shared__ double lds[];  // [9][TXP][TYP]
-    const Layout &L = p.lay;
-    const int TXP = g.tx + 2;
-    const int area = TXP * TYP;
+    extern __shared__ double lds[];  // [9][tx+2][TYP]
     const int by = blockIdx.x % g.nby;
     const int bx = blockIdx.x / g.nby;
-    const int Y0 = 1 + by * (TYP - 2);
-    const int X1 = L.lx + 1 - (g.nbx - 1 - bx) * g.tx;
-    const int X0 = X1 - g.tx > 1 ? X1 - g.tx : 1;
-    const int Y1 = Y0 + TYP - 2 < L.ly + 1 ? Y0 + TYP - 2 : L.ly + 1;
-
-    // step m on the tile plus its ring
-    for (int k = threadIdx.x; k < area; k += BLOCK_THREADS) {
-        int a = k / TYP, b = k % TYP;
-        int x = X0 - 1 + a, y = Y0 - 1 + b;
-        if (x > X1 || outside(L, x, y)) continue;
-        double f[9];
-        pull_node(L, P, x, y, f);
-        first_step(p, P, x, y, f);
-        for_pops([&]<int I>() { lds[I * area + k] = f[I]; });
-    }
-    lds_barrier();
-    // step m+1 on the tile
-    for (int k = threadIdx.x; k < area; k += BLOCK_THREADS) {
-        int a = k / TYP, b = k % TYP;
-        int x = X0 - 1 + a, y = Y0 - 1 + b;
-        if (x < X0 || x >= X1 || y < Y0 || y >= Y1) continue;
-        int64_t n = L.node(x, y);
-        double h[9];
-        for_pops([&]<int I>() {
-            int sx = x - cx(I), sy = y - cy(I);
-            h[I] = outside(L, sx, sy) ? P[I * L.plane + n] : lds[I * area + k - cx(I) * TYP - cy(I)];
-        });
-        int s = side_of(p, x, y);
-        double rp0 = 0.;
-        if (s >= 0) {
-            for_pops([&]<int I>() {
-                if constexpr (I != 0) {
-                    int gx = x - cx(I), gy = y - cy(I);
-                    if (outside(L, gx, gy)) Q[I * L.plane + L.node(gx, gy)] = lds[I * area + k];
-                }
-            });
-            if (s == RIGHT) {
-                // post-stream densities of step m+1 at columns lx-1, lx-2, from LDS
-                double g1[9], g2[9];
-                for_pops([&]<int I>() {
-                    g1[I] = lds[I * area + k - TYP - cx(I) * TYP - cy(I)];
-                    g2[I] = lds[I * area + k - 2 * TYP - cx(I) * TYP - cy(I)];
-                });
-                rp0 = zgb_rho(rho_of(g1), rho_of(g2));
-                p.outlet_rho[y] = p.zgb ? rp0 : p.side[RIGHT].p0[y];
-            }
-        }
-        collide_class(p, s, x, y, h, rp0);
-        for_pops([&]<int I>() { __builtin_nontemporal_store(h[I], &Q[I * L.plane + n]); });
-    }
+    block_tile<TYP>(p, g.tx, g.nbx, bx, by, P, Q, lds);
 }
 
 // ---- launchers ----------------------------------------------------------------
@@ -130,7 +33,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_block(Problem p, BlockGrid g,
 BlockGrid block_grid(const Layout &L, int tx, int typ) {
     BlockGrid g;
     if (typ != 32 && typ != 64 && typ != 128) typ = 64;
-    if (tx < 3) tx = 14;
+    if (tx < 3) tx = 8;   // 46 KB of LDS: three workgroups per CU
     // LDS: 9 populations x (tx+2) x typ doubles must fit 160 KB
     while (tx > 3 && 9 * (size_t)(tx + 2) * typ * 8 > 160 * 1024) --tx;
     g.tx = tx;

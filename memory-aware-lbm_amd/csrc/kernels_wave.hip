@@ -252,14 +252,6 @@ __device__ __forceinline__ S to_storage(typename Calc<S>::T v) {
     else return narrow<S>(v);
 }
 
-// Workgroups are dealt to the 8 XCDs in turn; renumber them so that each XCD
-// takes a contiguous range of tasks: neighbouring bands, whose halo rows and
-// shared cache lines overlap, then meet in the same L2.
-__device__ __forceinline__ int xcd_block(int b, int nblk) {
-    const int xcd = b % 8, k = b / 8, per = nblk / 8, rem = nblk % 8;
-    return xcd < rem ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
-}
-
 // Compile-time loop over K = 0 .. N-1.
 template <int N, typename F> __device__ __forceinline__ void static_for(F &&f) {
     [&]<int... K>(std::integer_sequence<int, K...>) __attribute__((always_inline)) {

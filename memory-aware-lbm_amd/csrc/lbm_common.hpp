@@ -81,4 +81,13 @@ __host__ __device__ inline int side_of(const Problem &p, int x, int y) {
     return -1;
 }
 
+// Workgroups are dealt to the 8 XCDs in turn; renumber them so that each XCD
+// takes a contiguous range of [0, nblk) (a bijection): neighbouring tiles or
+// bands, whose halo rows and shared cache lines overlap, then meet in the
+// same L2.
+__device__ __forceinline__ int xcd_block(int b, int nblk) {
+    const int xcd = b % 8, k = b / 8, per = nblk / 8, rem = nblk % 8;
+    return xcd < rem ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
+}
+
 }  // namespace lbm

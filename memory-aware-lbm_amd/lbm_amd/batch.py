@@ -8,8 +8,9 @@ y in [0.4, 0.6] ly, radius in [0.05, 0.12] ly (integer truncation, so short
 channels get radius 1-3), Re uniform in [20, 200], u_max 0.02.
 
 The autotuner times every (tile-height cap, workgroups per CU) pair of a small
-grid on the actual batch with hipGraph replay and keeps the fastest; results
-are cached in JSON keyed by storage format and batch shape.
+grid of the one-step launch and, for FP64, every tile width of the two-step
+launch, on the actual batch with hipGraph replay, and keeps the fastest;
+results are cached in JSON keyed by storage format and batch shape.
 """
 from __future__ import annotations
 
@@ -72,6 +73,7 @@ def _sig(lib):
         "lbm_hip_batch_destroy": (C.c_int, [vp]),
         "lbm_hip_batch_init_equilibrium": (C.c_int, [vp, C.POINTER(_P)]),
         "lbm_hip_batch_set_launch": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+        "lbm_hip_batch_set_depth": (C.c_int, [vp, C.c_int, C.c_int]),
         "lbm_hip_batch_step": (C.c_int, [vp, C.c_int]),
         "lbm_hip_batch_sync": (C.c_int, [vp]),
         "lbm_hip_batch_time": (C.c_int, [vp, C.c_int, _P]),
@@ -88,7 +90,8 @@ def _sig(lib):
 
 
 class Batch:
-    """Many channels advanced together by one persistent launch per step."""
+    """Many channels advanced together: one persistent launch per step, or
+    (FP64, the default there) one launch of two-step LDS tiles per two steps."""
 
     def __init__(self, channels: Iterable[Channel], storage: str = "f64"):
         lib = _sig(library())
@@ -100,7 +103,9 @@ class Batch:
         h = C.c_void_p()
         _check(lib.lbm_hip_batch_create(arr, len(self.channels), C.byref(h)), "batch_create")
         self._h = h
-        self.launch = {"wgs_per_cu": 8, "ty_cap": 256, "graph_steps": 0}
+        self.launch = {"wgs_per_cu": 8, "ty_cap": 256, "graph_steps": 0,
+                       "depth": 2 if storage == "f64" and min(c.lx for c in self.channels) >= 4
+                       else 1, "tile_cols": 8}
 
     def close(self):
         if getattr(self, "_h", None):
@@ -140,6 +145,14 @@ class Batch:
         if ty_cap:
             self.launch["ty_cap"] = ty_cap
         self.launch["graph_steps"] = graph_steps
+
+    def set_depth(self, depth: int, tile_cols: int = 0):
+        """Steps per launch (1, or 2 for FP64) and the two-step tile width."""
+        _check(library().lbm_hip_batch_set_depth(self._h, int(depth), int(tile_cols)),
+               "batch_set_depth")
+        self.launch["depth"] = depth
+        if tile_cols:
+            self.launch["tile_cols"] = tile_cols
 
     def step(self, n: int):
         _check(library().lbm_hip_batch_step(self._h, int(n)), "batch_step")
@@ -181,35 +194,49 @@ def batch_key(batch: Batch) -> str:
 
 
 def autotune(batch: Batch, steps: int = 20, graph_steps: int = 10,
-             ty_caps=(32, 64, 128, 256), wgs=(2, 4, 8, 16),
+             ty_caps=(32, 64, 128, 256), wgs=(2, 4, 8, 16), tile_cols=(8, 14, 22, 30),
              cache_dir: Optional[str] = TUNE_DIR, use_cache: bool = True) -> dict:
     """Pick the launch shape with the highest throughput on this batch.
 
     Every candidate runs `steps` steps (after one warm-up replay) with
     hipGraph replay; the state keeps advancing, which does not change the
-    per-step cost.  Returns {"ty_cap", "wgs_per_cu", "graph_steps", "glups",
+    per-step cost.  Candidates: the one-step launch over ty_caps x wgs and,
+    for FP64, the two-step launch over tile_cols.  Returns {"depth",
+    "tile_cols", "ty_cap", "wgs_per_cu", "graph_steps", "glups",
     "candidates": [...]}; cached by batch_key."""
     key = batch_key(batch)
     path = os.path.join(cache_dir, key + ".json") if cache_dir else None
     if use_cache and path and os.path.exists(path):
         best = json.load(open(path))
         batch.set_launch(best["wgs_per_cu"], best["ty_cap"], best["graph_steps"])
+        batch.set_depth(best.get("depth", 1), best.get("tile_cols", 0))
         best["cached"] = True
         return best
     cells = batch.cells
     results = []
+
+    def timed(**shape):
+        batch.time_steps(graph_steps)  # capture + warm
+        ms = batch.time_steps(steps)
+        results.append(dict(shape, glups=cells * steps / (ms * 1e-3) / 1e9))
+
+    batch.set_depth(1)
     for ty in ty_caps:
         for w in wgs:
             batch.set_launch(w, ty, graph_steps)
-            batch.time_steps(graph_steps)  # capture + warm
-            ms = batch.time_steps(steps)
-            results.append({"ty_cap": ty, "wgs_per_cu": w,
-                            "glups": cells * steps / (ms * 1e-3) / 1e9})
+            timed(depth=1, tile_cols=batch.launch["tile_cols"], ty_cap=ty, wgs_per_cu=w)
+    if batch.storage == "f64":
+        for tc in tile_cols:
+            batch.set_depth(2, tc)
+            timed(depth=2, tile_cols=tc, ty_cap=batch.launch["ty_cap"],
+                  wgs_per_cu=batch.launch["wgs_per_cu"])
     best = max(results, key=lambda r: r["glups"])
-    out = {"ty_cap": best["ty_cap"], "wgs_per_cu": best["wgs_per_cu"],
-           "graph_steps": graph_steps, "glups": best["glups"], "key": key,
+    out = {"depth": best["depth"], "tile_cols": best["tile_cols"], "ty_cap": best["ty_cap"],
+           "wgs_per_cu": best["wgs_per_cu"], "graph_steps": graph_steps,
+           "glups": best["glups"], "key": key,
            "tuned_at": time.strftime("%Y-%m-%d %H:%M:%S"), "candidates": results}
     batch.set_launch(out["wgs_per_cu"], out["ty_cap"], graph_steps)
+    batch.set_depth(out["depth"], out["tile_cols"])
     if path:
         os.makedirs(cache_dir, exist_ok=True)
         with open(path, "w") as fh:

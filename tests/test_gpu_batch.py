@@ -1,6 +1,7 @@
-"""Batches: every channel of a persistent batched launch evolves bit for bit
-as the same channel in its own context (and as the oracle), for every launch
-shape, with and without hipGraph replay."""
+"""Batches: every channel of a batched launch evolves bit for bit as the same
+channel in its own context (and as the oracle), for one and two steps per
+launch, every launch shape and tile width, with and without hipGraph
+replay."""
 import numpy as np
 import pytest
 
@@ -63,10 +64,60 @@ def test_launch_shape_does_not_change_results(ty_cap, wgs):
     with Batch(chans, "f64") as a, Batch(chans, "f64") as b:
         b.set_launch(wgs, ty_cap, 0)
         for x in (a, b):
+            x.set_depth(1)
             x.init_equilibrium()
             x.step(17)
         for i in range(len(chans)):
             assert np.array_equal(a.download(i)[0], b.download(i)[0])
+
+
+@pytest.mark.parametrize("tile_cols", [3, 8, 33])
+def test_two_steps_per_launch_equal_one(tile_cols):
+    """The two-step LDS tiles (the FP64 default) against the one-step
+    launch, over odd and even step counts and several tile widths (3 columns:
+    every channel's right end in a tile of its own width; 33: the widest
+    tile LDS holds)."""
+    chans = small_channels()
+    with Batch(chans, "f64") as one, Batch(chans, "f64") as two:
+        one.set_depth(1)
+        two.set_depth(2, tile_cols)
+        for x in (one, two):
+            x.init_equilibrium()
+        for chunk in (1, 2, 5, 12):
+            one.step(chunk)
+            two.step(chunk)
+            for i in range(len(chans)):
+                g, w = two.download(i), one.download(i)
+                ch = chans[i]
+                assert np.array_equal(g[0][1:-1, 1:-1], w[0][1:-1, 1:-1]), (chunk, i)
+                assert np.array_equal(g[1][1:-1, 1:-1], w[1][1:-1, 1:-1]), (chunk, i)
+                assert np.array_equal(g[2][2:ch.ly], w[2][2:ch.ly]), (chunk, i)
+        assert one.steps_done() == two.steps_done() == 20
+
+
+def test_two_steps_per_launch_need_fp64():
+    with Batch(small_channels()[:2], "f32") as b:
+        with pytest.raises(L.LbmError, match="FP64"):
+            b.set_depth(2)
+        with pytest.raises(L.LbmError, match="tile_cols"):
+            b.set_depth(1, 34)
+        b.set_depth(1)
+
+
+def test_narrow_channel_keeps_one_step_per_launch():
+    """A channel with lx = 3 has no room for the two-step tile's outlet
+    extrapolation (as the single-lattice two-step sweeps): such a batch
+    defaults to one step per launch, refuses two, and stays bit-exact."""
+    chans = small_channels()[:3] + [L.Channel(3, 5, 1, 2, 1)]
+    with Batch(chans, "f64") as b:
+        assert b.launch["depth"] == 1
+        with pytest.raises(L.LbmError, match="lx >= 4"):
+            b.set_depth(2)
+        b.init_equilibrium()
+        b.step(9)
+        got = [b.download(i)[0] for i in range(len(chans))]
+    for i, ch in enumerate(chans):
+        assert np.array_equal(got[i][1:-1, 1:-1], alone(ch, 9, "f64")[0][1:-1, 1:-1]), i
 
 
 @pytest.mark.parametrize("graph_steps", [1, 4, 7])
@@ -101,3 +152,16 @@ def test_seeded_workload_and_autotune(tmp_path):
         assert res["glups"] > 0 and len(res["candidates"]) == 4
         again = autotune(b, cache_dir=str(tmp_path))
         assert again.get("cached") and again["ty_cap"] == res["ty_cap"]
+        assert res["depth"] == 1
+
+
+def test_autotune_fp64_times_both_depths(tmp_path):
+    chans = channels_from_seed(16, seed=5)
+    with Batch(chans, "f64") as b:
+        b.init_equilibrium()
+        res = autotune(b, steps=4, graph_steps=3, ty_caps=(64,), wgs=(4,), tile_cols=(8, 14),
+                       cache_dir=str(tmp_path))
+        assert [c["depth"] for c in res["candidates"]] == [1, 2, 2]
+        assert res["depth"] in (1, 2) and b.launch["depth"] == res["depth"]
+        b.step(5)
+        assert b.steps_done() > 0
