@@ -40,13 +40,13 @@ sys.path.insert(0, os.path.join(ROOT, "memory-aware-lbm_amd"))
 HBM_PEAK_GBS = 8000.0
 METRIC = "GLUPS & % of HBM roofline (FP64 vs FP32-store); serving p50/p99 latency"
 KERNEL_OF = {"origin": "k_collide<false>", "combine": "k_collide<true>",
-             "tight": "k_collide<true>", "tight_block": "k_block<64>",
+             "tight": "k_collide<true>", "tight_block": "k_block<double, 64>",
              "panel": "k_wpanel_in<double, 2, 2>", "panel4": "k_wpanel_in<double, 1, 4>",
              "panel6": "k_wpanel_in<double, 2, 6>"}
 LOWP_KERNEL_OF = {(s, v): k.replace("double", {"f32": "float", "f16": "_Float16"}[s])
                   for s in ("f32", "f16") for v, k in
-                  (("combine", "k_collide_lp<double, true>"), ("panel", "k_wpanel_in<double, 2, 2>"),
-                   ("panel4", "k_wpanel_in<double, 2, 4>"))}
+                  (("combine", "k_collide_lp<double, true>"), ("tight_block", "k_block<double, 64>"),
+                   ("panel", "k_wpanel_in<double, 2, 2>"), ("panel4", "k_wpanel_in<double, 2, 4>"))}
 UPDATES_PER_LAUNCH = {"origin": 0.5, "combine": 1, "tight": 1, "tight_block": 2, "panel": 2,
                       "panel4": 4, "panel6": 6}
 PROFILE = os.path.join(ROOT, "profiles", "kernels_16384x4096.md")
@@ -64,12 +64,14 @@ def parse():
                     help="variants timed besides the headline one (comma list, or '')")
     ap.add_argument("--other-steps", type=int, default=1000)
     ap.add_argument("--storages", default="f32,f16",
-                    help="reduced-precision storage forms, timed with the fused and the "
-                         "panel sweeps")
+                    help="reduced-precision storage forms, timed with the fused, the "
+                         "tight_block and the panel sweeps")
     ap.add_argument("--batch", type=int, default=512,
                     help="channels of the batching workload (0 = skip)")
     ap.add_argument("--batch-seed", type=int, default=0)
     ap.add_argument("--batch-steps", type=int, default=100)
+    ap.add_argument("--batch-tune", default="on", choices=["on", "off"],
+                    help="autotune the batch launch shape (off: the library's defaults)")
     ap.add_argument("--serve", type=int, default=40,
                     help="requests of the serving workload (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrivals per second")
@@ -116,21 +118,25 @@ def measured_traffic(variant):
 
 def batching(args):
     """The batching workload: seeded uneven channels, hipGraph replay,
-    autotuned launch shape (FP64: two steps per launch over LDS tiles, or one
-    persistent launch per step; FP32: one per step); FP64 and FP32."""
+    autotuned launch shape (two steps per launch over LDS tiles, or one
+    persistent launch per step); FP64, FP32 and FP16 storage."""
     from lbm_amd.batch import Batch, autotune, channels_from_seed
     chans = channels_from_seed(args.batch, seed=args.batch_seed)
     out = {"channels": len(chans), "seed": args.batch_seed,
            "cells": int(sum(c.lx * c.ly for c in chans)),
            "tau_below_0505": int(sum(1 for c in chans if 1.0 / c.omega() < 0.505))}
-    for storage in ("f64", "f32"):
+    for storage in ("f64", "f32", "f16"):
         with Batch(chans, storage) as b:
             b.init_equilibrium()
-            tune = autotune(b, steps=10, graph_steps=10, use_cache=False)
+            if args.batch_tune == "on":
+                tune = autotune(b, steps=10, graph_steps=10, use_cache=False)
+            else:
+                b.set_launch(0, 0, 10)
+                tune = dict(b.launch)
             b.step(10)
             ms = b.time_steps(args.batch_steps)
             glups = b.cells * args.batch_steps / (ms * 1e-3) / 1e9
-            bpu = 144.0 if storage == "f64" else 72.0
+            bpu = {"f64": 144.0, "f32": 72.0, "f16": 36.0}[storage]
             out[storage] = {"glups": round(glups, 4), "ms_per_step": round(ms / args.batch_steps, 5),
                             "achieved_gbs": round(glups * bpu, 1),
                             "frac": round(glups * bpu / HBM_PEAK_GBS, 4),
@@ -236,7 +242,7 @@ def main():
     for sname in [v for v in args.storages.split(",") if v]:
         bpu = {"f32": 72.0, "f16": 36.0}[sname]
         storage[sname] = {"bytes_per_update": bpu}
-        for v in ("combine", "panel", "panel4"):
+        for v in ("combine", "tight_block", "panel", "panel4"):
             r = time_variant(lbm, ch, v, args.other_steps, min(args.warmup, 20), sname)
             ach = bpu * cells * UPDATES_PER_LAUNCH[v] / (r["launch_ms"] * 1e-3) / 1e9
             storage[sname][v] = {"glups": round(r["glups"], 4),

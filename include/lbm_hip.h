@@ -235,7 +235,7 @@ int lbm_hip_copy_bandwidth(size_t bytes, int iters, double *ms_per_copy);
  * channels must use the same storage format.  A step of the batch is one
  * persistent launch over a table of 256-node tiles of every channel; each
  * node runs the single-lattice node step, so every channel evolves bit for
- * bit as it would in its own context (fused pull variant).  FP64 batches
+ * bit as it would in its own context (fused pull variant).  Batches
  * advance two steps per launch by default (lbm_hip_batch_set_depth), with
  * the LDS tiles of the tight_block sweep: the same bits. */
 typedef struct lbm_batch lbm_batch;
@@ -251,8 +251,9 @@ int lbm_hip_batch_init_equilibrium(lbm_batch *b, const double *const *init_ux);
 int lbm_hip_batch_set_launch(lbm_batch *b, int wgs_per_cu, int ty_cap, int graph_steps);
 /* Steps per launch: 1 (the persistent one-step launch above) or 2 (one
  * workgroup per tile of tile_cols columns x 62 rows of every channel, the
- * first step in LDS; FP64 only and every channel's lx >= 4,
- * LBM_ERR_UNSUPPORTED otherwise; the default for such batches).  tile_cols in [3, 33] (default 8), 0 keeps it.  Odd
+ * first step in LDS; needs lx >= 4 in every channel, LBM_ERR_UNSUPPORTED
+ * otherwise; the default for such batches).  tile_cols in [3, 33] for FP64
+ * (default 8), [3, 69] for FP32 / FP16 (default 14 / 10); 0 keeps it.  Odd
  * remainders and the first step after init_equilibrium run one step. */
 int lbm_hip_batch_set_depth(lbm_batch *b, int depth, int tile_cols);
 int lbm_hip_batch_step(lbm_batch *b, int nsteps);

@@ -12,8 +12,8 @@
 // (node_step.hpp), so every channel's result is bit-identical to running it
 // alone.  K consecutive steps can be captured into a hipGraph and replayed.
 //
-// FP64 batches can instead advance two steps per launch (depth 2): one
-// workgroup per two-step LDS tile of the single-lattice tight_block sweep
+// Batches can instead advance two steps per launch (depth 2, the default):
+// one workgroup per two-step LDS tile of the single-lattice tight_block sweep
 // (block_tile.hpp), the tiles of all channels in one flat table, renumbered
 // so each XCD takes a contiguous range.  The same tile body, so the same
 // bits; the lattice crosses HBM once per two steps.
@@ -46,8 +46,10 @@ struct BatchEntry {
 
 constexpr int BATCH_TYP = 64;  // two-step tile rows + the one-node ring
 
-inline size_t batch_block_lds(int tile_cols) {
-    return 9 * (size_t)(tile_cols + 2) * BATCH_TYP * sizeof(double);
+// LDS of one two-step tile: compute-type values (8 bytes for FP64 storage,
+// 4 for FP32 / FP16)
+inline size_t batch_block_lds(int tile_cols, int storage) {
+    return 9 * (size_t)(tile_cols + 2) * BATCH_TYP * (storage == LBM_STORE_F64 ? 8 : 4);
 }
 
 template <typename S, bool PULL>
@@ -75,19 +77,26 @@ __global__ __launch_bounds__(256) void k_batch(const BatchEntry *__restrict__ en
     }
 }
 
-// Two steps of every channel, one workgroup per tile (FP64, collided state
-// in A, collided state two steps later in B).
+// Two steps of every channel, one workgroup per tile (collided state in A,
+// collided state two steps later in B).
+template <typename S>
 __global__ __launch_bounds__(BLOCK_THREADS) void k_batch_block(const BatchEntry *__restrict__ ents,
                                                              const int *__restrict__ tile_chan,
-                                                             const double *__restrict__ A,
-                                                             double *__restrict__ B) {
-    extern __shared__ double lds[];  // [9][btx+2][BATCH_TYP]
+                                                             const S *__restrict__ A,
+                                                             S *__restrict__ B) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];  // [9][btx+2][BATCH_TYP]
     const int t = xcd_block(blockIdx.x, gridDim.x);
     const int c = __builtin_amdgcn_readfirstlane(tile_chan[t]);
     const BatchEntry &e = ents[c];
     const int local = t - (int)e.tile0;
-    const int by = local % e.bnby, bx = local / e.bnby;
-    block_tile<BATCH_TYP>(e.p, e.btx, e.bnbx, bx, by, A + e.off, B + e.off, lds);
+    int by = local % e.bnby, bx = local / e.bnby;
+    if constexpr (sizeof(S) == 4) {
+        pin_index(by);
+        pin_index(bx);
+    }
+    const uint32_t m = sizeof(S) == 4 ? *e.p.step : 0u;   // the launch's first collision
+    block_tile<S, BATCH_TYP>(e.p, e.btx, e.bnbx, bx, by, m, A + e.off, B + e.off,
+                             (typename Calc<S>::T *)lds_raw);
 }
 
 template <typename S>
@@ -134,8 +143,9 @@ struct lbm_batch {
     int graph_steps = 0;
     hipGraphExec_t graph[2] = {nullptr, nullptr};  // by starting buffer
     int graph_flip[2] = {0, 0};                     // buffer swaps per replay, mod 2
-    int depth = 1;                                  // steps per launch (2: FP64 tiles)
-    int tile_cols = 8;                              // columns of a two-step tile
+    int depth = 1;                                  // steps per launch (2: LDS tiles)
+    int tile_cols = 8;                              // columns of a two-step tile (set
+                                                    // per storage format at creation)
     int min_lx = 0;                                 // two-step tiles need lx >= 4
     int *d_tile_chan = nullptr;                     // channel of each two-step tile
     int64_t ntiles = 0;
@@ -219,11 +229,28 @@ hipError_t launch_step(lbm_batch *b, bool pull) {
     return e == hipSuccess ? launch_bump(b->d_step, 1, b->stream) : e;
 }
 
-// two steps of every channel from the collided state (FP64 only)
+// two steps of every channel from the collided state
 hipError_t launch_pair(lbm_batch *b) {
-    k_batch_block<<<(unsigned)b->ntiles, BLOCK_THREADS, batch_block_lds(b->tile_cols), b->stream>>>(
-        b->d_ents, b->d_tile_chan, (const double *)b->pool[b->cur], (double *)b->pool[1 - b->cur]);
-    return hipGetLastError();
+    const dim3 grid((unsigned)b->ntiles);
+    const size_t lds = batch_block_lds(b->tile_cols, b->storage);
+    const void *src = b->pool[b->cur];
+    void *dst = b->pool[1 - b->cur];
+    switch (b->storage) {
+    case LBM_STORE_F64:
+        k_batch_block<double><<<grid, BLOCK_THREADS, lds, b->stream>>>(
+            b->d_ents, b->d_tile_chan, (const double *)src, (double *)dst);
+        return hipGetLastError();
+    case LBM_STORE_F32:
+        k_batch_block<float><<<grid, BLOCK_THREADS, lds, b->stream>>>(
+            b->d_ents, b->d_tile_chan, (const float *)src, (float *)dst);
+        break;
+    default:
+        k_batch_block<_Float16><<<grid, BLOCK_THREADS, lds, b->stream>>>(
+            b->d_ents, b->d_tile_chan, (const _Float16 *)src, (_Float16 *)dst);
+        break;
+    }
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? launch_bump(b->d_step, 2, b->stream) : e;
 }
 
 // steps of the next launch: two where the depth and the state allow it
@@ -317,7 +344,9 @@ int lbm_hip_batch_create(const lbm_problem *probs, int n, lbm_batch **out) {
     b->storage = storage;
     b->min_lx = probs[0].lx;
     for (int c = 1; c < n; ++c) b->min_lx = std::min(b->min_lx, probs[c].lx);
-    b->depth = storage == LBM_STORE_F64 && b->min_lx >= 4 ? 2 : 1;
+    b->depth = b->min_lx >= 4 ? 2 : 1;
+    // fastest on the 512-channel workload: LDS for 3 / 4 / 5 workgroups per CU
+    b->tile_cols = storage == LBM_STORE_F64 ? 8 : storage == LBM_STORE_F32 ? 14 : 10;
     b->ents.resize(n);
     int dev = 0;
     hipDeviceProp_t prop;
@@ -351,8 +380,10 @@ int lbm_hip_batch_create(const lbm_problem *probs, int n, lbm_batch **out) {
     };
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return bail(e, "hipStreamCreate");
-    e = hipFuncSetAttribute((const void *)k_batch_block,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    for (const void *k : {(const void *)k_batch_block<double>, (const void *)k_batch_block<float>,
+                          (const void *)k_batch_block<_Float16>})
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (e != hipSuccess) return bail(e, "hipFuncSetAttribute");
     for (int k = 0; k < 2; ++k) {
         e = hipMalloc(&b->pool[k], (size_t)off * esize(storage));
@@ -479,10 +510,8 @@ int lbm_hip_batch_set_launch(lbm_batch *b, int wgs_per_cu, int ty_cap, int graph
 int lbm_hip_batch_set_depth(lbm_batch *b, int depth, int tile_cols) {
     if (!b || (depth != 1 && depth != 2) || tile_cols < 0)
         return fail(LBM_ERR_ARG, "bad depth setting");
-    if (tile_cols > 0 && (tile_cols < 3 || batch_block_lds(tile_cols) > 160 * 1024))
-        return fail(LBM_ERR_ARG, "tile_cols must be in [3, 33]");
-    if (depth == 2 && b->storage != LBM_STORE_F64)
-        return fail(LBM_ERR_UNSUPPORTED, "two steps per launch need FP64 storage");
+    if (tile_cols > 0 && (tile_cols < 3 || batch_block_lds(tile_cols, b->storage) > 160 * 1024))
+        return fail(LBM_ERR_ARG, "tile_cols must be in [3, 33] (FP64) or [3, 69] (FP32, FP16)");
     // the outlet extrapolation of a tile's second step reads the first step
     // of column lx-3 (as the single-lattice two-step sweeps)
     if (depth == 2 && b->min_lx < 4)

@@ -276,8 +276,6 @@ int step_impl(lbm_ctx *c, int variant, int nsteps) {
         // with one fused single step
         if (c->prob.lay.lx < 4)
             return fail(LBM_ERR_UNSUPPORTED, "two-step variants need lx >= 4");
-        if (variant == LBM_TIGHT_BLOCK && c->storage != LBM_STORE_F64)
-            return fail(LBM_ERR_UNSUPPORTED, "tight_block is built for FP64 storage only");
         int k = 0;
         if (!c->collided) {
             int rc = to_collided(c);
@@ -302,9 +300,11 @@ int step_impl(lbm_ctx *c, int variant, int nsteps) {
                 HIPCHK(panel_launch(c, c->cur));
             } else {
                 BlockGrid g = block_grid(c->prob.lay, c->tile_x,
-                                         c->tile_y > 0 ? c->tile_y + 2 : 0);
-                HIPCHK(launch_block_f64(c->prob, g, dbuf(c, c->cur), dbuf(c, 1 - c->cur),
-                                        c->stream));
+                                         c->tile_y > 0 ? c->tile_y + 2 : 0,
+                                         c->storage == LBM_STORE_F64 ? 8 : 4);
+                HIPCHK(launch_block(c->storage, c->prob, g, c->buf[c->cur], c->buf[1 - c->cur],
+                                    c->stream));
+                if (c->storage != LBM_STORE_F64) HIPCHK(launch_bump(c->d_step, 2, c->stream));
             }
             c->cur = 1 - c->cur;
             c->steps += 2;

@@ -38,75 +38,14 @@
 #include <utility>
 
 #include "launch.hpp"
-#include "fields.hpp"
-#include "node_step.hpp"
+#include "step_any.hpp"
 
 namespace lbm {
-
-template <typename S> struct Calc { using T = float; };
-template <> struct Calc<double> { using T = double; };
-
-template <typename S> __device__ __forceinline__ typename Calc<S>::T as_calc(S v) {
-    return (typename Calc<S>::T)v;
-}
-
-// round a computed value to what storage would hold, back in compute type
-template <typename S> __device__ __forceinline__ typename Calc<S>::T stored(typename Calc<S>::T v) {
-    if constexpr (sizeof(S) == 2) return (float)narrow<S>(v);
-    else return v;
-}
 
 __device__ __forceinline__ double lane_up(double v) { return __shfl_up(v, 1, 64); }
 __device__ __forceinline__ double lane_down(double v) { return __shfl_down(v, 1, 64); }
 __device__ __forceinline__ float lane_up(float v) { return __shfl_up(v, 1, 64); }
 __device__ __forceinline__ float lane_down(float v) { return __shfl_down(v, 1, 64); }
-
-// density of compute-type populations, summed in FP64 the way the
-// single-step kernels sum what they load
-template <typename S> __device__ __forceinline__ double rho_of_any(const typename Calc<S>::T h[9]) {
-    if constexpr (sizeof(S) == 8) {
-        return rho_of(h);
-    } else {
-        double f[9];
-        for_pops([&]<int I>() { f[I] = wgt(I) + (double)h[I]; });
-        return rho_of(f);
-    }
-}
-
-// post-stream density of interior node (x, y) gathered from P
-template <typename S>
-__device__ __forceinline__ double pulled_rho(const Layout &L, const S *__restrict__ P, int x, int y) {
-    double f[9];
-    int64_t n = L.node(x, y);
-    for_pops([&]<int I>() {
-        f[I] = pop_f64<S, I>(&P[I * L.plane + n - cx(I) * L.pitch - cy(I)]);
-    });
-    return rho_of(f);
-}
-
-// One node's collision on compute-type populations h, side s (-1: interior);
-// `step` is the collision's index (it keys FP32's stochastic rounding, so the
-// first step held in registers rounds exactly as a stored single step would).
-template <typename S>
-__device__ __forceinline__ void collide_any(const Problem &p, int s, int x, int y,
-                                            typename Calc<S>::T h[9], double right_p0,
-                                            uint32_t step) {
-    // the panel kernels run FP32 storage with toward-zero FP32 rounding
-    if constexpr (sizeof(S) == 8) {
-        collide_class(p, s, x, y, h, right_p0);
-    } else {
-        const SrSeed hs = sizeof(S) == 4 ? sr_seed(p.lay.node(x, y), step) : SrSeed{0u, 0u};
-        if (s < 0) {
-            if (p.r2 >= 0 && in_disc(p, x, y)) bounce_dev(h);
-            else bgk_lp<S, true>(h, p.omega, hs);
-            return;
-        }
-        double f[9];
-        for_pops([&]<int I>() { f[I] = wgt(I) + (double)h[I]; });
-        collide_class(p, s, x, y, f, right_p0);
-        for_pops([&]<int I>() { h[I] = to_dev<S, I, true>(f[I], hs); });
-    }
-}
 
 // ---- perimeter waves ---------------------------------------------------------
 // Waves whose rows or columns (halo included) touch the channel perimeter
@@ -244,12 +183,6 @@ __device__ __forceinline__ void buffer_store(S v, __amdgpu_buffer_rsrc_t r, uint
         __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, off, 0, CPOL_STORE);
     else
         __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(uint16_t, v), r, off, 0, CPOL_STORE);
-}
-
-template <typename S>
-__device__ __forceinline__ S to_storage(typename Calc<S>::T v) {
-    if constexpr (sizeof(S) == 8) return v;
-    else return narrow<S>(v);
 }
 
 // Compile-time loop over K = 0 .. N-1.

@@ -44,14 +44,15 @@ hipError_t launch_narrow(int storage, int64_t plane, const double *src, void *ds
 
 // Two-step (temporally blocked) kernels: kernels_2step_f64.hip
 struct BlockGrid {
-    int tx;   // tile columns (>= 3)
-    int typ;  // tile rows + 2 (32, 64 or 128)
+    int tx;      // tile columns (>= 3)
+    int typ;     // tile rows + 2 (32, 64 or 128)
+    int tbytes;  // bytes per LDS value (8: FP64 storage, 4: FP32 / FP16 storage)
     int nbx, nby;
 };
-BlockGrid block_grid(const Layout &L, int tx, int typ);
+BlockGrid block_grid(const Layout &L, int tx, int typ, int tbytes);
 size_t block_lds_bytes(const BlockGrid &g);
-hipError_t launch_block_f64(const Problem &p, const BlockGrid &g, const double *P, double *Q,
-                            hipStream_t st);
+hipError_t launch_block(int storage, const Problem &p, const BlockGrid &g, const void *P, void *Q,
+                        hipStream_t st);
 hipError_t block_kernel_setup();
 
 // Two steps per launch, first step held in wave64 registers: kernels_wave.hip

@@ -8,7 +8,7 @@ y in [0.4, 0.6] ly, radius in [0.05, 0.12] ly (integer truncation, so short
 channels get radius 1-3), Re uniform in [20, 200], u_max 0.02.
 
 The autotuner times every (tile-height cap, workgroups per CU) pair of a small
-grid of the one-step launch and, for FP64, every tile width of the two-step
+grid of the one-step launch and every tile width of a list for the two-step
 launch, on the actual batch with hipGraph replay, and keeps the fastest;
 results are cached in JSON keyed by storage format and batch shape.
 """
@@ -90,8 +90,8 @@ def _sig(lib):
 
 
 class Batch:
-    """Many channels advanced together: one persistent launch per step, or
-    (FP64, the default there) one launch of two-step LDS tiles per two steps."""
+    """Many channels advanced together: one launch of two-step LDS tiles per
+    two steps (the default), or one persistent launch per step."""
 
     def __init__(self, channels: Iterable[Channel], storage: str = "f64"):
         lib = _sig(library())
@@ -104,8 +104,8 @@ class Batch:
         _check(lib.lbm_hip_batch_create(arr, len(self.channels), C.byref(h)), "batch_create")
         self._h = h
         self.launch = {"wgs_per_cu": 8, "ty_cap": 256, "graph_steps": 0,
-                       "depth": 2 if storage == "f64" and min(c.lx for c in self.channels) >= 4
-                       else 1, "tile_cols": 8}
+                       "depth": 2 if min(c.lx for c in self.channels) >= 4 else 1,
+                       "tile_cols": {"f64": 8, "f32": 14, "f16": 10}[storage]}
 
     def close(self):
         if getattr(self, "_h", None):
@@ -147,7 +147,7 @@ class Batch:
         self.launch["graph_steps"] = graph_steps
 
     def set_depth(self, depth: int, tile_cols: int = 0):
-        """Steps per launch (1, or 2 for FP64) and the two-step tile width."""
+        """Steps per launch (1 or 2) and the two-step tile width."""
         _check(library().lbm_hip_batch_set_depth(self._h, int(depth), int(tile_cols)),
                "batch_set_depth")
         self.launch["depth"] = depth
@@ -194,14 +194,16 @@ def batch_key(batch: Batch) -> str:
 
 
 def autotune(batch: Batch, steps: int = 20, graph_steps: int = 10,
-             ty_caps=(32, 64, 128, 256), wgs=(2, 4, 8, 16), tile_cols=(8, 14, 22, 30),
+             ty_caps=(32, 64, 128, 256), wgs=(2, 4, 8, 16), tile_cols=None,
              cache_dir: Optional[str] = TUNE_DIR, use_cache: bool = True) -> dict:
     """Pick the launch shape with the highest throughput on this batch.
 
     Every candidate runs `steps` steps (after one warm-up replay) with
     hipGraph replay; the state keeps advancing, which does not change the
-    per-step cost.  Candidates: the one-step launch over ty_caps x wgs and,
-    for FP64, the two-step launch over tile_cols.  Returns {"depth",
+    per-step cost.  Candidates: the one-step launch over ty_caps x wgs and
+    the two-step launch over tile_cols (default: widths around three and two
+    workgroups per CU for the storage format; skipped when a channel is
+    narrower than four columns).  Returns {"depth",
     "tile_cols", "ty_cap", "wgs_per_cu", "graph_steps", "glups",
     "candidates": [...]}; cached by batch_key."""
     key = batch_key(batch)
@@ -225,7 +227,9 @@ def autotune(batch: Batch, steps: int = 20, graph_steps: int = 10,
         for w in wgs:
             batch.set_launch(w, ty, graph_steps)
             timed(depth=1, tile_cols=batch.launch["tile_cols"], ty_cap=ty, wgs_per_cu=w)
-    if batch.storage == "f64":
+    if tile_cols is None:
+        tile_cols = (6, 8, 10, 14) if batch.storage == "f64" else (10, 14, 18)
+    if min(c.lx for c in batch.channels) >= 4:
         for tc in tile_cols:
             batch.set_depth(2, tc)
             timed(depth=2, tile_cols=tc, ty_cap=batch.launch["ty_cap"],

@@ -28,7 +28,7 @@ def alone(ch, steps, storage):
         return lat.download()
 
 
-@pytest.mark.parametrize("storage", ["f64", "f32"])
+@pytest.mark.parametrize("storage", ["f64", "f32", "f16"])
 def test_batch_equals_single_contexts(storage):
     """Bit-identical for every storage format: the batched and the
     single-lattice kernels run the same node step."""
@@ -71,14 +71,16 @@ def test_launch_shape_does_not_change_results(ty_cap, wgs):
             assert np.array_equal(a.download(i)[0], b.download(i)[0])
 
 
-@pytest.mark.parametrize("tile_cols", [3, 8, 33])
-def test_two_steps_per_launch_equal_one(tile_cols):
-    """The two-step LDS tiles (the FP64 default) against the one-step
-    launch, over odd and even step counts and several tile widths (3 columns:
-    every channel's right end in a tile of its own width; 33: the widest
-    tile LDS holds)."""
+@pytest.mark.parametrize("storage,tile_cols", [("f64", 3), ("f64", 8), ("f64", 33),
+                                               ("f32", 3), ("f32", 14), ("f32", 69),
+                                               ("f16", 10), ("f16", 40)])
+def test_two_steps_per_launch_equal_one(storage, tile_cols):
+    """The two-step LDS tiles (the default) against the one-step launch,
+    over odd and even step counts and several tile widths (3 columns: every
+    channel's right end in a tile of its own width; 33 / 69: the widest tile
+    LDS holds with FP64 / FP32 values)."""
     chans = small_channels()
-    with Batch(chans, "f64") as one, Batch(chans, "f64") as two:
+    with Batch(chans, storage) as one, Batch(chans, storage) as two:
         one.set_depth(1)
         two.set_depth(2, tile_cols)
         for x in (one, two):
@@ -95,13 +97,16 @@ def test_two_steps_per_launch_equal_one(tile_cols):
         assert one.steps_done() == two.steps_done() == 20
 
 
-def test_two_steps_per_launch_need_fp64():
-    with Batch(small_channels()[:2], "f32") as b:
-        with pytest.raises(L.LbmError, match="FP64"):
-            b.set_depth(2)
+def test_two_step_tile_width_limits():
+    with Batch(small_channels()[:2], "f64") as b, Batch(small_channels()[:2], "f16") as h:
         with pytest.raises(L.LbmError, match="tile_cols"):
-            b.set_depth(1, 34)
-        b.set_depth(1)
+            b.set_depth(2, 34)
+        with pytest.raises(L.LbmError, match="tile_cols"):
+            h.set_depth(2, 70)
+        with pytest.raises(L.LbmError, match="tile_cols"):
+            h.set_depth(2, 2)
+        b.set_depth(2, 33)
+        h.set_depth(2, 69)
 
 
 def test_narrow_channel_keeps_one_step_per_launch():
@@ -148,14 +153,16 @@ def test_seeded_workload_and_autotune(tmp_path):
     with Batch(chans, "f32") as b:
         b.init_equilibrium()
         res = autotune(b, steps=4, graph_steps=2, ty_caps=(64, 256), wgs=(4, 8),
-                       cache_dir=str(tmp_path))
-        assert res["glups"] > 0 and len(res["candidates"]) == 4
+                       tile_cols=(18,), cache_dir=str(tmp_path))
+        assert res["glups"] > 0 and len(res["candidates"]) == 5
+        assert [c["depth"] for c in res["candidates"]] == [1, 1, 1, 1, 2]
         again = autotune(b, cache_dir=str(tmp_path))
         assert again.get("cached") and again["ty_cap"] == res["ty_cap"]
-        assert res["depth"] == 1
+        assert again["depth"] == res["depth"] and b.launch["depth"] == res["depth"]
 
 
 def test_autotune_fp64_times_both_depths(tmp_path):
+    """The two-step candidates of FP64 widths; the state keeps advancing."""
     chans = channels_from_seed(16, seed=5)
     with Batch(chans, "f64") as b:
         b.init_equilibrium()

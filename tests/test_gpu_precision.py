@@ -108,12 +108,22 @@ def test_reduced_upload_download_round_trip(orc, storage, tol):
     assert np.abs(to[1:-1, 1:-1] - tref[1:-1, 1:-1]).max() <= tol
 
 
-def test_tight_block_reduced_is_refused():
-    ch = L.Channel.geometry(64, 32, "B")
-    with L.Lattice(ch, storage="f32") as lat:
-        lat.init_equilibrium()
-        with pytest.raises(L.LbmError):
-            lat.step(4, "tight_block")
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+@pytest.mark.parametrize("shape", [(97, 41, 0, 0), (300, 140, 5, 30), (64, 200, 33, 126),
+                                   (5, 7, 3, 0), (300, 260, 69, 62), (301, 260, 20, 0)])
+def test_tight_block_reduced_equals_single_steps(storage, shape):
+    """Two steps per launch with the first step in LDS tiles: the first
+    step's values are rounded to the storage format before they go to LDS, so
+    the result equals single steps bit for bit, for every tile shape (tile_x
+    columns x tile_y rows; 0 = default)."""
+    lx, ly, tx, ty = shape
+    ch = L.Channel(lx, ly, lx // 5, ly // 2, max(ly // 10, 1))
+    with L.Lattice(ch, storage=storage) as a, L.Lattice(ch, storage=storage) as b:
+        b.set_tile(tx, ty)
+        a.init_equilibrium(); b.init_equilibrium()
+        a.step(31, "combine"); b.step(31, "tight_block")
+        la, ta, pa = a.download(); lb, tb, pb = b.download()
+    assert np.array_equal(la, lb) and np.array_equal(ta, tb) and np.array_equal(pa, pb)
 
 
 @pytest.mark.parametrize("storage", ["f32", "f16"])
