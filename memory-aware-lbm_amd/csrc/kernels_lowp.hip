@@ -14,7 +14,7 @@
 // ring carrying the unknown populations) are those of kernels_f64.hip.
 //
 // No counterpart in the reference (its arithmetic is FP64 throughout); the
-// error against the FP64 path is measured, not assumed (tests/).
+// error against the FP64 path is measured, not assumed (tests/test_gpu_precision.py).
 #include "launch.hpp"
 #include "collide_f64.hpp"
 #include "fields.hpp"
