@@ -77,7 +77,9 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrivals per second")
     ap.add_argument("--serve-seed", type=int, default=0)
     ap.add_argument("--cpu-baseline", default="auto", choices=["auto", "off"])
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=150,
+                    help="steps of each reference driver (150: about 10-15 s of CPU "
+                         "work each on 16 cores at 16384x4096)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     return ap.parse_args()
 
