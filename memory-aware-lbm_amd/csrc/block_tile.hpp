@@ -32,12 +32,6 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// Integer division expands into FP32 reciprocal arithmetic, which must not
-// run under toward-zero FP32 rounding: a tile index computed by division is
-// passed through here before the rounding mode changes, and the side-effect
-// order keeps the division ahead of the switch.
-__device__ __forceinline__ void pin_index(int &v) { __asm__ volatile("" : "+v"(v)); }
-
 // Tile (bx, by) of a lattice cut into nbx right-aligned column strips of tx
 // columns and rows of TYP-2; lds holds 9 x (tx+2) x TYP compute-type values;
 // m is the index of the tile's first collision (FP32 rounding key).

@@ -21,7 +21,8 @@
 //
 // Waves whose rows or columns touch the channel perimeter run a separate,
 // general kernel (k_wpanel_edge: wall operators, outlet extrapolation,
-// unknown populations); the interior kernel (k_wpanel_in) runs BGK /
+// unknown populations; for the deep sweeps also the thin strips along the
+// walls, eight chunks per wave); the interior kernel (k_wpanel_in) runs BGK /
 // bounce-back only, branch-free.  The second step's outlet density is
 // extrapolated from the post-stream densities of columns lx-2 and lx-1
 // gathered by the same lane just before their second collision
@@ -67,20 +68,33 @@ __device__ __forceinline__ void edge_task(const WaveGrid &g, int e, int &band, i
     }
 }
 
-// dstep: index of this launch's first collision relative to the counter
-template <typename S>
-__global__ __launch_bounds__(64) void k_wpanel_edge(Problem p, WaveGrid g, uint32_t dstep,
-                                                    const S *__restrict__ P, S *__restrict__ Q) {
+// One perimeter wave's two-step march over its chunk(s).  THIN: eight groups
+// of eight lanes, group k on chunk ck + k of a strip of at most six rows
+// (rows Y0 .. Y0+nrows-1 on lanes 1 .. nrows of the group, halo rows on
+// lanes 0 and nrows+1: a group's owned rows read shifted values from their
+// own group only); otherwise one chunk, rows Y0 .. Y0+nrows-1 on lanes
+// 1 .. nrows.  dstep: index of this launch's first collision relative to
+// the counter.
+template <typename S, bool THIN>
+__device__ __forceinline__ void edge_march(const Problem &p, const WaveGrid &g, uint32_t dstep,
+                                           const S *__restrict__ P, S *__restrict__ Q, int Y0,
+                                           int nrows, int ck) {
     using T = typename Calc<S>::T;
     const Layout &L = p.lay;
+    if constexpr (sizeof(S) == 4) {
+        pin_index(Y0);
+        pin_index(nrows);
+        pin_index(ck);
+    }
     const int lane = threadIdx.x;
-    int band, ck;
-    edge_task(g, blockIdx.x, band, ck);
-    const WaveSpan w = wave_span(L, g, band, ck);
-    const int Y0 = w.Y0, X0 = w.X0, X1 = w.X1;
-    const int y = Y0 - 1 + lane;
-    const bool row_in = y >= 1 && y <= L.ly;
-    const bool owner = lane >= 1 && lane <= WAVE_ROWS && y <= L.ly;
+    const int lr = THIN ? (lane & 7) : lane;             // row Y0 - 1 + lr
+    const int ckl = THIN ? ck + (lane >> 3) : ck;
+    const bool active = ckl < g.ncx;                     // THIN: a strip's last wave may run short
+    const WaveSpan w = wave_span(L, g, 0, active ? ckl : g.ncx - 1);
+    const int X0 = w.X0, X1 = w.X1;
+    const int y = Y0 - 1 + lr;
+    const bool row_in = active && y >= 1 && y <= L.ly;
+    const bool owner = active && lr >= 1 && lr <= nrows && y <= L.ly;
     const int s_row = (y == 1) ? LOWER : (y == L.ly) ? UPPER : -1;
     const uint32_t m = sizeof(S) == 4 ? *p.step + dstep : 0u;   // index of the first collision
 
@@ -97,10 +111,10 @@ __global__ __launch_bounds__(64) void k_wpanel_edge(Problem p, WaveGrid g, uint3
     if (row_in && X0 - 1 >= 1) pull(X0 - 1, fn);
     if constexpr (sizeof(S) == 4) fp32_round_toward_zero();
 
-    for (int x = X0 - 1; x <= X1; ++x) {
+    auto column = [&](int x) {
         // ---- step m, column x, all 64 rows -> registers
         for_pops([&]<int I>() { cur[I] = as_calc<S>(fn[I]); });
-        if (row_in && x + 1 <= X1 && x + 1 <= L.lx) pull(x + 1, fn);
+        if (row_in && x + 1 >= 1 && x + 1 <= X1 && x + 1 <= L.lx) pull(x + 1, fn);
         if (row_in && x >= 1 && x <= L.lx) {
             int s = s_row >= 0 ? s_row : (x == 1 ? LEFT : (x == L.lx ? RIGHT : -1));
             double rp0 = 0.;
@@ -148,7 +162,44 @@ __global__ __launch_bounds__(64) void k_wpanel_edge(Problem p, WaveGrid g, uint3
         c2[1] = c1[5];
         c2[2] = c1[8];
         for_pops([&]<int I>() { c1[I] = cur[I]; });
+    };
+    if constexpr (THIN) {
+        // the same cx + 2 iterations for every group: a group's march starts
+        // at X1 - cx - 1, below X0 - 1 for a partial first chunk, whose extra
+        // iterations neither load nor compute
+        const int xbeg = X1 - g.cx - 1;
+        for (int k = 0; k <= g.cx + 1; ++k) column(xbeg + k);
+    } else {
+        for (int x = X0 - 1; x <= X1; ++x) column(x);
     }
+}
+
+template <typename S>
+__global__ __launch_bounds__(64) void k_wpanel_edge(Problem p, WaveGrid g, uint32_t dstep,
+                                                    const S *__restrict__ P, S *__restrict__ Q) {
+    int e = blockIdx.x;
+    if (g.thin) {
+        // waves [0, nthin): the strips below r0 and from r1 up, eight chunks
+        // each; then the edge chunks of rows [r0, r1) in 62-row bands
+        const int nps = g.nthin / 2;
+        if (e < g.nthin) {
+            const bool top = e >= nps;
+            const int ck0 = (top ? e - nps : e) * 8;
+            edge_march<S, true>(p, g, dstep, P, Q, top ? g.r1 : 1,
+                                top ? p.lay.ly + 1 - g.r1 : g.r0 - 1, ck0);
+            return;
+        }
+        e -= g.nthin;
+        const int ns = g.c_lo + (g.ncx - g.c_hi);
+        const int i = e / ns, j = e % ns;
+        const int Y0 = g.r0 + i * WAVE_ROWS;
+        edge_march<S, false>(p, g, dstep, P, Q, Y0, g.r1 - Y0 < WAVE_ROWS ? g.r1 - Y0 : WAVE_ROWS,
+                             j < g.c_lo ? j : g.c_hi + (j - g.c_lo));
+        return;
+    }
+    int band, ck;
+    edge_task(g, e, band, ck);
+    edge_march<S, false>(p, g, dstep, P, Q, 1 + band * WAVE_ROWS, WAVE_ROWS, ck);
 }
 
 // ---- interior waves -------------------------------------------------------------
@@ -370,15 +421,23 @@ static WaveGrid wave_grid_max(const Layout &L, int cx, int cmax, int64_t waves) 
     g.r0 = 1 + b_lo * WAVE_ROWS;
     g.r1 = 1 + b_hi * WAVE_ROWS;
     g.nbl = b_hi - b_lo;
+    g.thin = g.nthin = 0;
     return g;
 }
 
 WaveGrid wave_grid(const Layout &L, int cx) { return wave_grid_max(L, cx, 16, 16384); }
 
 // The grid of the deep sweeps (LEVELS = 4 or 6 steps per launch, H = LEVELS-1
-// halo rows and columns): the interior rectangle of the two-step grid, shrunk
-// until its H-node halo stays off the perimeter (rows and columns 2 .. ly-1 /
-// lx-1), tiled by (64 - 2H)-row bands (the last one partial).  The deep
+// halo rows and columns): the interior chunks of the two-step grid, shrunk
+// until their H-column halo stays off columns 1 and lx, over rows r0 = H + 2
+// .. r1 = ly - H, the rows whose H-row halo stays within rows 2 .. ly - 1,
+// tiled by (64 - 2H)-row bands (the last one partial).  The frame is then
+// H + 1 rows deep along the walls: its perimeter waves take eight chunks of
+// such a strip each (eight lanes per chunk), so a pass has few perimeter
+// waves: 196 on 16384 x 4096 against 896 for a frame of whole 62-row bands.
+// Perimeter waves beyond about one per CU keep the interior kernel's
+// workgroups off their CUs; with the thin frame the six-step FP64 sweep
+// runs 2% faster, the four-step FP32 / FP16 ones 3%.  The deep
 // march recomputes 2H halo columns per chunk and does LEVELS/2 times the
 // two-step arithmetic per column, so it wants wider chunks, up to 24, and
 // fewer waves (about 6144) before narrowing them: four-step FP64 GLUPS at 4 /
@@ -401,32 +460,36 @@ static WaveGrid wave_grid_deep(const Layout &L, int cx, int storage, int levels)
     WaveGrid g = wave_grid_max(L, cx, cmax, 6144);
     const int H = levels - 1, rows = 64 - 2 * H;
     auto chunk_x0 = [&](int ck) { return L.lx + 1 - (g.ncx - 1 - ck) * g.cx - g.cx; };
-    while (g.b_hi > g.b_lo && 1 + g.b_hi * WAVE_ROWS - 1 + H > L.ly - 1) --g.b_hi;
-    while (g.b_lo < g.b_hi && 1 + g.b_lo * WAVE_ROWS - H < 2) ++g.b_lo;
     while (g.c_lo < g.c_hi && chunk_x0(g.c_lo) - H < 2) ++g.c_lo;
     while (g.c_hi > g.c_lo && chunk_x0(g.c_hi - 1) + g.cx - 1 + H > L.lx - 1) --g.c_hi;
-    if (g.b_lo >= g.b_hi || g.c_lo >= g.c_hi) {
+    // rows: as close to the walls as the halo allows (rows r0 - H .. r1 - 1 + H
+    // within 2 .. ly - 1), so the frame keeps H + 1 rows along each wall
+    g.r0 = 2 + H;
+    g.r1 = L.ly - H;
+    if (g.c_lo >= g.c_hi || g.r1 - g.r0 < 2 * WAVE_ROWS) {
+        // no interior for the deep march: every wave is a perimeter wave
         g.b_lo = g.b_hi = g.c_lo = g.c_hi = 0;
         g.nin = 0;
         g.nedge = g.ntasks;
         g.nbl = 0;
         return g;
     }
-    g.r0 = 1 + g.b_lo * WAVE_ROWS;
-    g.r1 = 1 + g.b_hi * WAVE_ROWS;
+    g.thin = 1;
+    g.b_lo = g.b_hi = 0;   // the thin frame has no (band, chunk) rectangle
     g.nbl = (g.r1 - g.r0 + rows - 1) / rows;
     g.nin = g.nbl * (g.c_hi - g.c_lo);
-    g.nedge = g.ntasks - (g.b_hi - g.b_lo) * (g.c_hi - g.c_lo);
+    g.nthin = 2 * ((g.ncx + 7) / 8);
+    g.nedge = g.nthin + (g.r1 - g.r0 + WAVE_ROWS - 1) / WAVE_ROWS * (g.c_lo + g.ncx - g.c_hi);
     return g;
 }
 
 bool wpanel_deep_possible(const Layout &L, int cx, int storage, int levels) {
     if (levels != 4 && levels != 6) return false;
     WaveGrid g = wave_grid_deep(L, cx, storage, levels);
-    // an interior of at least two bands and two chunks (it then holds the
+    // an interior of at least 124 rows and two chunks (it then holds the
     // rings of the perimeter passes, at most LEVELS - 2 nodes thick, many
     // times over)
-    return g.b_hi - g.b_lo >= 2 && g.c_hi - g.c_lo >= 2;
+    return g.thin && g.c_hi - g.c_lo >= 2;
 }
 
 // Perimeter waves on `side`, interior waves on `st`, joined back into `st`:

@@ -12,6 +12,12 @@
 
 namespace lbm {
 
+// Integer division expands into FP32 reciprocal arithmetic, which must not
+// run under toward-zero FP32 rounding (fp32_round_toward_zero): an index
+// computed by division is passed through here before the rounding mode
+// changes, and the side-effect order keeps the division ahead of the switch.
+__device__ __forceinline__ void pin_index(int &v) { __asm__ volatile("" : "+v"(v)); }
+
 template <typename S> struct Calc { using T = float; };
 template <> struct Calc<double> { using T = double; };
 
