@@ -425,7 +425,41 @@ static WaveGrid wave_grid_max(const Layout &L, int cx, int cmax, int64_t waves) 
     return g;
 }
 
-WaveGrid wave_grid(const Layout &L, int cx) { return wave_grid_max(L, cx, 16, 16384); }
+// The thin frame of a grid whose interior waves keep an H-row / H-column
+// halo: interior chunks shrunk until the halo stays off columns 1 and lx,
+// interior rows r0 = H + 2 .. r1 = ly - H (the rows whose halo stays within
+// rows 2 .. ly - 1) in (64 - 2H)-row bands, the last one partial.  The frame
+// keeps H + 1 rows along each wall, taken eight chunks per perimeter wave,
+// and the edge chunks of rows [r0, r1) in 62-row bands.  Fewer than
+// min_rows interior rows or no interior chunk: every wave is a perimeter
+// wave.
+static WaveGrid make_thin(const Layout &L, WaveGrid g, int H, int min_rows) {
+    const int rows = 64 - 2 * H;
+    auto chunk_x0 = [&](int ck) { return L.lx + 1 - (g.ncx - 1 - ck) * g.cx - g.cx; };
+    while (g.c_lo < g.c_hi && chunk_x0(g.c_lo) - H < 2) ++g.c_lo;
+    while (g.c_hi > g.c_lo && chunk_x0(g.c_hi - 1) + g.cx - 1 + H > L.lx - 1) --g.c_hi;
+    g.r0 = 2 + H;
+    g.r1 = L.ly - H;
+    if (g.c_lo >= g.c_hi || g.r1 - g.r0 < min_rows) {
+        g.b_lo = g.b_hi = g.c_lo = g.c_hi = 0;
+        g.nin = 0;
+        g.nedge = g.ntasks;
+        g.nbl = 0;
+        g.thin = g.nthin = 0;
+        return g;
+    }
+    g.thin = 1;
+    g.b_lo = g.b_hi = 0;   // the thin frame has no (band, chunk) rectangle
+    g.nbl = (g.r1 - g.r0 + rows - 1) / rows;
+    g.nin = g.nbl * (g.c_hi - g.c_lo);
+    g.nthin = 2 * ((g.ncx + 7) / 8);
+    g.nedge = g.nthin + (g.r1 - g.r0 + WAVE_ROWS - 1) / WAVE_ROWS * (g.c_lo + g.ncx - g.c_hi);
+    return g;
+}
+
+WaveGrid wave_grid(const Layout &L, int cx) {
+    return make_thin(L, wave_grid_max(L, cx, 16, 16384), 1, 1);
+}
 
 // The grid of the deep sweeps (LEVELS = 4 or 6 steps per launch, H = LEVELS-1
 // halo rows and columns): the interior chunks of the two-step grid, shrunk
@@ -434,10 +468,11 @@ WaveGrid wave_grid(const Layout &L, int cx) { return wave_grid_max(L, cx, 16, 16
 // tiled by (64 - 2H)-row bands (the last one partial).  The frame is then
 // H + 1 rows deep along the walls: its perimeter waves take eight chunks of
 // such a strip each (eight lanes per chunk), so a pass has few perimeter
-// waves: 196 on 16384 x 4096 against 896 for a frame of whole 62-row bands.
-// Perimeter waves beyond about one per CU keep the interior kernel's
-// workgroups off their CUs; with the thin frame the six-step FP64 sweep
-// runs 2% faster, the four-step FP32 / FP16 ones 3%.  The deep
+// waves: 196 on 16384 x 4096 against 896 for a frame of whole 62-row bands
+// (the two-step grid: 388 against 2178).  Perimeter waves beyond about one
+// per CU keep the interior kernel's workgroups off their CUs; with the thin
+// frame the six-step FP64 sweep runs 2-3% faster, the four-step FP32 / FP16
+// ones 3-6%, the two-step FP32 / FP16 ones 3-4%.  The deep
 // march recomputes 2H halo columns per chunk and does LEVELS/2 times the
 // two-step arithmetic per column, so it wants wider chunks, up to 24, and
 // fewer waves (about 6144) before narrowing them: four-step FP64 GLUPS at 4 /
@@ -457,38 +492,15 @@ static WaveGrid wave_grid_deep(const Layout &L, int cx, int storage, int levels)
     const bool largest = bands * L.lx >= 128 * 6144;
     const int cmax = !largest ? 24 : levels == 6 ? 64 : storage != 2 ? 40 : 24;
     if (cx < 3 && largest && levels == 4 && storage == 0) cx = 63;
-    WaveGrid g = wave_grid_max(L, cx, cmax, 6144);
-    const int H = levels - 1, rows = 64 - 2 * H;
-    auto chunk_x0 = [&](int ck) { return L.lx + 1 - (g.ncx - 1 - ck) * g.cx - g.cx; };
-    while (g.c_lo < g.c_hi && chunk_x0(g.c_lo) - H < 2) ++g.c_lo;
-    while (g.c_hi > g.c_lo && chunk_x0(g.c_hi - 1) + g.cx - 1 + H > L.lx - 1) --g.c_hi;
-    // rows: as close to the walls as the halo allows (rows r0 - H .. r1 - 1 + H
-    // within 2 .. ly - 1), so the frame keeps H + 1 rows along each wall
-    g.r0 = 2 + H;
-    g.r1 = L.ly - H;
-    if (g.c_lo >= g.c_hi || g.r1 - g.r0 < 2 * WAVE_ROWS) {
-        // no interior for the deep march: every wave is a perimeter wave
-        g.b_lo = g.b_hi = g.c_lo = g.c_hi = 0;
-        g.nin = 0;
-        g.nedge = g.ntasks;
-        g.nbl = 0;
-        return g;
-    }
-    g.thin = 1;
-    g.b_lo = g.b_hi = 0;   // the thin frame has no (band, chunk) rectangle
-    g.nbl = (g.r1 - g.r0 + rows - 1) / rows;
-    g.nin = g.nbl * (g.c_hi - g.c_lo);
-    g.nthin = 2 * ((g.ncx + 7) / 8);
-    g.nedge = g.nthin + (g.r1 - g.r0 + WAVE_ROWS - 1) / WAVE_ROWS * (g.c_lo + g.ncx - g.c_hi);
-    return g;
+    // at least two bands of interior rows: they hold the rings of the
+    // perimeter passes, at most LEVELS - 2 nodes thick, many times over
+    return make_thin(L, wave_grid_max(L, cx, cmax, 6144), levels - 1, 2 * WAVE_ROWS);
 }
 
 bool wpanel_deep_possible(const Layout &L, int cx, int storage, int levels) {
     if (levels != 4 && levels != 6) return false;
     WaveGrid g = wave_grid_deep(L, cx, storage, levels);
-    // an interior of at least 124 rows and two chunks (it then holds the
-    // rings of the perimeter passes, at most LEVELS - 2 nodes thick, many
-    // times over)
+    // an interior of at least 124 rows and two chunks
     return g.thin && g.c_hi - g.c_lo >= 2;
 }
 
