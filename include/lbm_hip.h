@@ -214,8 +214,14 @@ double lbm_hip_bytes_per_node(const lbm_ctx *ctx, int variant);
 
 /* Total device memory the context holds, in bytes. */
 size_t lbm_hip_footprint(const lbm_ctx *ctx);
-/* Device memory a problem would need, without allocating. */
+/* Device memory a problem would need, without allocating: the two
+ * population buffers every variant uses. */
 size_t lbm_hip_footprint_of(const lbm_problem *prob);
+/* The same for stepping it with `variant`: LBM_PANEL4 adds the one further
+ * population buffer it allocates on first use, LBM_PANEL6 two (an upper
+ * bound: a lattice too small for the deep march runs the shallower sweep and
+ * allocates none).  No counterpart in the reference (admission support). */
+size_t lbm_hip_footprint_for(const lbm_problem *prob, int variant);
 
 /* Known-answer entry: apply one node operator of the device to n population
  * vectors (9 doubles each, in place, host memory).  which follows the order

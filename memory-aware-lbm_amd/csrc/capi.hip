@@ -354,6 +354,13 @@ size_t lbm_hip_footprint_of(const lbm_problem *p) {
     return pops + sides + (size_t)(p->ly + 2) * sizeof(double) + diag;
 }
 
+size_t lbm_hip_footprint_for(const lbm_problem *p, int variant) {
+    const size_t base = lbm_hip_footprint_of(p);
+    if (!base) return 0;
+    const size_t extra = variant == LBM_PANEL6 ? 2 : variant == LBM_PANEL4 ? 1 : 0;
+    return base + extra * 9 * (size_t)make_layout(p->lx, p->ly).plane * elem_size(p->storage);
+}
+
 int lbm_hip_create(const lbm_problem *p, lbm_ctx **out) {
     if (!out) return fail(LBM_ERR_ARG, "null output pointer");
     *out = nullptr;

@@ -79,6 +79,7 @@ def library():
         "lbm_hip_bytes_per_node": (C.c_double, [vp, C.c_int]),
         "lbm_hip_footprint": (C.c_size_t, [vp]),
         "lbm_hip_footprint_of": (C.c_size_t, [C.POINTER(_Problem)]),
+        "lbm_hip_footprint_for": (C.c_size_t, [C.POINTER(_Problem), C.c_int]),
         "lbm_hip_apply_node_op": (C.c_int, [C.c_int, C.c_int, _P, _P, _P, C.c_double]),
         "lbm_hip_copy_bandwidth": (C.c_int, [C.c_size_t, C.c_int, _P]),
         "lbm_hip_set_graph_steps": (C.c_int, [vp, C.c_int]),
@@ -273,11 +274,14 @@ def device_memory():
     return int(f.value), int(t.value)
 
 
-def footprint_of(ch: "Channel", storage: str = "f64") -> int:
-    """Device bytes a context for this channel would hold (no allocation)."""
+def footprint_of(ch: "Channel", storage: str = "f64", variant: Optional[str] = None) -> int:
+    """Device bytes a context for this channel would hold (no allocation);
+    with a variant, including the further buffers that sweep allocates."""
     p = _Problem()
     p.lx, p.ly, p.storage = ch.lx, ch.ly, STORAGE[storage]
-    return int(library().lbm_hip_footprint_of(C.byref(p)))
+    if variant is None:
+        return int(library().lbm_hip_footprint_of(C.byref(p)))
+    return int(library().lbm_hip_footprint_for(C.byref(p), VARIANT[variant]))
 
 
 def copy_bandwidth(nbytes: int, iters: int = 20) -> float:

@@ -6,15 +6,16 @@ their own clock (a seeded Poisson stream for the serving workload).  The
 server loop:
 
   * admits waiting requests first-fit, in arrival order, as long as a slot is
-    free (max_active) and the device-memory footprint of the new lattice fits
-    the HBM budget next to the solves already resident;
+    free (max_active) and the device-memory footprint of the new lattice
+    (with the further buffers of the sweep it will run) fits the HBM budget
+    next to the solves already resident;
   * keeps every admitted lattice resident on the device until its solve ends;
   * hands each resident solve a time slice of steps (about slice_ms of
     work, a multiple of the captured graph length) on the solve's own HIP
     stream whenever its previous slice has finished, so solves of different
     sizes run concurrently on the GPU and a newly admitted request starts at
-    the next slice boundary instead of waiting for the queue to drain; large
-    lattices run the panel sweeps, small ones the fused sweep;
+    the next slice boundary instead of waiting for the queue to drain; each
+    solve runs the sweep fastest for its size and format (SolveServer.SWEEPS);
   * on completion takes the solve's diagnostics (and, if asked, its velocity
     field), frees its memory and records arrival -> completion latency.
 
@@ -104,13 +105,23 @@ class SolveServer:
     """Admission by device-memory footprint, device-resident lattices,
     time-sliced concurrent execution (one HIP stream per solve)."""
 
-    # variant="auto" picks by lattice size: the fused sweep below
-    # PANEL_MIN_CELLS, the two-step panel up to PANEL4_MIN_CELLS, the
-    # four-step panel above.  Measured on MI355X (FP64 GLUPS, fused / panel /
-    # panel4): 1024 x 256 22.8 / 15.3 / 15.5, 2048 x 512 31.6 / 38.5 / 36,
-    # 4096 x 1024 36.8 / 53.4 / 66, 8192 x 2048 36.6 / 55.2 / 86.
-    PANEL_MIN_CELLS = 1_500_000
-    PANEL4_MIN_CELLS = 3_000_000
+    # variant="auto" picks the sweep that is fastest for one solve of that
+    # size and format: SWEEPS lists (minimum cells, variant) in increasing
+    # size.  Measured on MI355X, one solve alone, graphs of 50 steps, GLUPS
+    # fused / tight_block / panel / panel4 / panel6:
+    #   FP64 512 x 128 11.0 / 9.5 / 4.2 / 4.1 / 4.1, 1024 x 256 22.5 / 23.6 /
+    #   15.1 / 13.4 / 14.2, 1536 x 384 24.7 / 34.5 / 32.4 / 24.6 / 24.7,
+    #   2048 x 512 32.1 / 41.3 / 44.4 / 37.9 / 36.8, 3072 x 768 36.1 / 49.7 /
+    #   51.0 / 54.8 / 59.6, 4096 x 1024 37.2 / 53.9 / 52.5 / 67.9 / 73.1,
+    #   8192 x 2048 36.8 / 52.8 / 57.4 / 99.1 / 105.3;
+    #   FP32 (no panel6) 1024 x 256 29.5 / 16.6 / 16.0 / 10.8, 2048 x 512
+    #   48.5 / 41.5 / 45.8 / 36.7, 3072 x 768 61.2 / 59.7 / 73.1 / 63.2,
+    #   4096 x 1024 65.8 / 72.3 / 83.4 / 78.1, 8192 x 2048 66.6 / 76.5 /
+    #   107.2 / 120.3.
+    SWEEPS = {"f64": ((0, "combine"), (400_000, "tight_block"), (1_000_000, "panel"),
+                      (2_000_000, "panel6")),
+              "f32": ((0, "combine"), (1_500_000, "panel"), (8_000_000, "panel4")),
+              "f16": ((0, "combine"), (1_500_000, "panel"), (8_000_000, "panel4"))}
 
     def __init__(self, hbm_budget: Optional[int] = None, max_active: int = 8,
                  slice_ms: float = 20.0, graph_steps: int = 50,
@@ -132,9 +143,7 @@ class SolveServer:
         if self.variant != "auto":
             return self.variant
         cells = req.channel.lx * req.channel.ly
-        if cells >= self.PANEL4_MIN_CELLS:
-            return "panel4"
-        return "panel" if cells >= self.PANEL_MIN_CELLS else "combine"
+        return [v for lo, v in self.SWEEPS[req.storage] if cells >= lo][-1]
 
     def _slice(self, req: Request) -> int:
         cells = req.channel.lx * req.channel.ly
@@ -158,7 +167,7 @@ class SolveServer:
             for req in list(waiting):
                 if len(active) >= self.max_active:
                     break
-                fp = footprint_of(req.channel, req.storage)
+                fp = footprint_of(req.channel, req.storage, self._variant(req))
                 if fp > self.hbm_budget:
                     raise ValueError(f"request {req.rid} needs {fp} B, budget {self.hbm_budget} B")
                 if used + fp > self.hbm_budget:

@@ -65,6 +65,21 @@ def test_footprint_without_allocating():
     assert 9.6e9 < b < 1.05e10
 
 
+def test_footprint_of_a_sweep_counts_its_buffers():
+    """Admission support: the four- and six-step sweeps allocate one and two
+    further population buffers of 9 planes of (lx + 2) x pitch elements."""
+    import lbm_amd
+    for lx, ly, st, es in ((2048, 1024, "f64", 8), (300, 77, "f32", 4), (64, 5, "f16", 2)):
+        ch = lbm_amd.Channel.geometry(lx, ly, "B")
+        base = lbm_amd.footprint_of(ch, st)
+        pitch = (63 + ly + 2 + 63) // 64 * 64
+        buf = 9 * (lx + 2) * pitch * es
+        assert lbm_amd.footprint_of(ch, st, "combine") == base
+        assert lbm_amd.footprint_of(ch, st, "panel") == base
+        assert lbm_amd.footprint_of(ch, st, "panel4") == base + buf
+        assert lbm_amd.footprint_of(ch, st, "panel6") == base + 2 * buf
+
+
 def test_unbound_sweep_aborts_without_gpu():
     """A sweep on a Simulation that was never bound to a device aborts with a
     message instead of computing anything on the CPU."""

@@ -53,6 +53,22 @@ def test_admission_budget_and_slots():
     assert s["p99_latency_s"] >= s["p50_latency_s"] > 0
 
 
+def test_admission_counts_the_sweeps_buffers():
+    """A 2048 x 1024 FP64 solve runs the six-step sweep, whose two further
+    buffers the admission counts; the context then holds exactly that."""
+    ch = L.Channel.geometry(2048, 1024, "B")
+    req = Request(0, 0.0, ch, 12, "f64")
+    srv = SolveServer(max_active=1, slice_ms=5.0, graph_steps=20)
+    assert srv._variant(req) == "panel6"
+    srv.run([req])
+    want = L.footprint_of(ch, "f64", "panel6")
+    assert srv.stats["peak_bytes"] == want > L.footprint_of(ch, "f64")
+    with L.Lattice(ch) as lat:
+        lat.init_equilibrium()
+        lat.step(12, "panel6")
+        assert lat.footprint() == want
+
+
 def test_late_arrival_overlaps_running_solve():
     big = Request(0, 0.0, L.Channel(2048, 512, 400, 256, 40), 20000, "f64")
     small = Request(1, 0.05, L.Channel(64, 16, 12, 8, 2), 200, "f32")
