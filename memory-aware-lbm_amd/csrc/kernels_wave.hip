@@ -68,18 +68,18 @@ __device__ __forceinline__ void edge_task(const WaveGrid &g, int e, int &band, i
     }
 }
 
-// One perimeter wave's two-step march over its chunk(s).  THIN: eight groups
-// of eight lanes, group k on chunk ck + k of a strip of at most six rows
-// (rows Y0 .. Y0+nrows-1 on lanes 1 .. nrows of the group, halo rows on
-// lanes 0 and nrows+1: a group's owned rows read shifted values from their
-// own group only); otherwise one chunk, rows Y0 .. Y0+nrows-1 on lanes
-// 1 .. nrows.  dstep: index of this launch's first collision relative to
+// One perimeter wave's two-step march over its chunk(s).  GL < 64: groups of
+// GL lanes, group k on chunk ck + k of a strip of at most GL - 2 rows (rows
+// Y0 .. Y0+nrows-1 on lanes 1 .. nrows of the group, halo rows on lanes 0 and
+// nrows+1: a group's owned rows read shifted values from their own group
+// only); GL = 64: one chunk, rows Y0 .. Y0+nrows-1 on lanes 1 .. nrows.  dstep: index of this launch's first collision relative to
 // the counter.
-template <typename S, bool THIN>
+template <typename S, int GL>
 __device__ __forceinline__ void edge_march(const Problem &p, const WaveGrid &g, uint32_t dstep,
                                            const S *__restrict__ P, S *__restrict__ Q, int Y0,
                                            int nrows, int ck) {
     using T = typename Calc<S>::T;
+    constexpr bool THIN = GL < 64;
     const Layout &L = p.lay;
     if constexpr (sizeof(S) == 4) {
         pin_index(Y0);
@@ -87,8 +87,8 @@ __device__ __forceinline__ void edge_march(const Problem &p, const WaveGrid &g, 
         pin_index(ck);
     }
     const int lane = threadIdx.x;
-    const int lr = THIN ? (lane & 7) : lane;             // row Y0 - 1 + lr
-    const int ckl = THIN ? ck + (lane >> 3) : ck;
+    const int lr = lane & (GL - 1);                      // row Y0 - 1 + lr
+    const int ckl = ck + lane / GL;
     const bool active = ckl < g.ncx;                     // THIN: a strip's last wave may run short
     const WaveSpan w = wave_span(L, g, 0, active ? ckl : g.ncx - 1);
     const int X0 = w.X0, X1 = w.X1;
@@ -174,32 +174,34 @@ __device__ __forceinline__ void edge_march(const Problem &p, const WaveGrid &g, 
     }
 }
 
-template <typename S>
+// GL: lanes per chunk of the thin strips (g.gl; 4 for the two-step grid's
+// two-row strips, 8 for the deep grids' strips of up to six rows)
+template <typename S, int GL>
 __global__ __launch_bounds__(64) void k_wpanel_edge(Problem p, WaveGrid g, uint32_t dstep,
                                                     const S *__restrict__ P, S *__restrict__ Q) {
     int e = blockIdx.x;
     if (g.thin) {
-        // waves [0, nthin): the strips below r0 and from r1 up, eight chunks
-        // each; then the edge chunks of rows [r0, r1) in 62-row bands
+        // waves [0, nthin): the strips below r0 and from r1 up, 64 / GL
+        // chunks each; then the edge chunks of rows [r0, r1) in 62-row bands
         const int nps = g.nthin / 2;
         if (e < g.nthin) {
             const bool top = e >= nps;
-            const int ck0 = (top ? e - nps : e) * 8;
-            edge_march<S, true>(p, g, dstep, P, Q, top ? g.r1 : 1,
-                                top ? p.lay.ly + 1 - g.r1 : g.r0 - 1, ck0);
+            const int ck0 = (top ? e - nps : e) * (64 / GL);
+            edge_march<S, GL>(p, g, dstep, P, Q, top ? g.r1 : 1,
+                              top ? p.lay.ly + 1 - g.r1 : g.r0 - 1, ck0);
             return;
         }
         e -= g.nthin;
         const int ns = g.c_lo + (g.ncx - g.c_hi);
         const int i = e / ns, j = e % ns;
         const int Y0 = g.r0 + i * WAVE_ROWS;
-        edge_march<S, false>(p, g, dstep, P, Q, Y0, g.r1 - Y0 < WAVE_ROWS ? g.r1 - Y0 : WAVE_ROWS,
-                             j < g.c_lo ? j : g.c_hi + (j - g.c_lo));
+        edge_march<S, 64>(p, g, dstep, P, Q, Y0, g.r1 - Y0 < WAVE_ROWS ? g.r1 - Y0 : WAVE_ROWS,
+                          j < g.c_lo ? j : g.c_hi + (j - g.c_lo));
         return;
     }
     int band, ck;
     edge_task(g, e, band, ck);
-    edge_march<S, false>(p, g, dstep, P, Q, 1 + band * WAVE_ROWS, WAVE_ROWS, ck);
+    edge_march<S, 64>(p, g, dstep, P, Q, 1 + band * WAVE_ROWS, WAVE_ROWS, ck);
 }
 
 // ---- interior waves -------------------------------------------------------------
@@ -421,7 +423,7 @@ static WaveGrid wave_grid_max(const Layout &L, int cx, int cmax, int64_t waves) 
     g.r0 = 1 + b_lo * WAVE_ROWS;
     g.r1 = 1 + b_hi * WAVE_ROWS;
     g.nbl = b_hi - b_lo;
-    g.thin = g.nthin = 0;
+    g.thin = g.nthin = g.gl = 0;
     return g;
 }
 
@@ -429,8 +431,9 @@ static WaveGrid wave_grid_max(const Layout &L, int cx, int cmax, int64_t waves) 
 // halo: interior chunks shrunk until the halo stays off columns 1 and lx,
 // interior rows r0 = H + 2 .. r1 = ly - H (the rows whose halo stays within
 // rows 2 .. ly - 1) in (64 - 2H)-row bands, the last one partial.  The frame
-// keeps H + 1 rows along each wall, taken eight chunks per perimeter wave,
-// and the edge chunks of rows [r0, r1) in 62-row bands.  Fewer than
+// keeps H + 1 rows along each wall, taken 64 / gl chunks per perimeter wave
+// (gl = 4 lanes per chunk for H = 1, 8 for H = 3 and 5), and the edge chunks
+// of rows [r0, r1) in 62-row bands.  Fewer than
 // min_rows interior rows or no interior chunk: every wave is a perimeter
 // wave.
 static WaveGrid make_thin(const Layout &L, WaveGrid g, int H, int min_rows) {
@@ -445,14 +448,16 @@ static WaveGrid make_thin(const Layout &L, WaveGrid g, int H, int min_rows) {
         g.nin = 0;
         g.nedge = g.ntasks;
         g.nbl = 0;
-        g.thin = g.nthin = 0;
+        g.thin = g.nthin = g.gl = 0;
         return g;
     }
     g.thin = 1;
     g.b_lo = g.b_hi = 0;   // the thin frame has no (band, chunk) rectangle
     g.nbl = (g.r1 - g.r0 + rows - 1) / rows;
     g.nin = g.nbl * (g.c_hi - g.c_lo);
-    g.nthin = 2 * ((g.ncx + 7) / 8);
+    g.gl = H + 3 <= 4 ? 4 : 8;   // H + 1 strip rows and two halo rows per chunk
+    const int cpw = 64 / g.gl;
+    g.nthin = 2 * ((g.ncx + cpw - 1) / cpw);
     g.nedge = g.nthin + (g.r1 - g.r0 + WAVE_ROWS - 1) / WAVE_ROWS * (g.c_lo + g.ncx - g.c_hi);
     return g;
 }
@@ -469,7 +474,8 @@ WaveGrid wave_grid(const Layout &L, int cx) {
 // H + 1 rows deep along the walls: its perimeter waves take eight chunks of
 // such a strip each (eight lanes per chunk), so a pass has few perimeter
 // waves: 196 on 16384 x 4096 against 896 for a frame of whole 62-row bands
-// (the two-step grid: 388 against 2178).  Perimeter waves beyond about one
+// (the two-step grid, sixteen chunks of its two-row strips per wave: 260
+// against 2178).  Perimeter waves beyond about one
 // per CU keep the interior kernel's workgroups off their CUs; with the thin
 // frame the six-step FP64 sweep runs 2-3% faster, the four-step FP32 / FP16
 // ones 3-6%, the two-step FP32 / FP16 ones 3-4%.  The deep
@@ -518,7 +524,10 @@ hipError_t launch_wpanel(int storage, const Problem &p, const WaveGrid &g, const
     if (e != hipSuccess) return e;
     auto go = [&](auto tag) {
         using S = decltype(tag);
-        if (g.nedge) k_wpanel_edge<S><<<g.nedge, 64, 0, side>>>(p, g, 0u, (const S *)P, (S *)Q);
+        if (g.nedge && g.gl == 4)
+            k_wpanel_edge<S, 4><<<g.nedge, 64, 0, side>>>(p, g, 0u, (const S *)P, (S *)Q);
+        else if (g.nedge)
+            k_wpanel_edge<S, 8><<<g.nedge, 64, 0, side>>>(p, g, 0u, (const S *)P, (S *)Q);
         if (!g.nin) return;
         const S *src = (const S *)P;
         S *dst = (S *)Q;
@@ -619,7 +628,7 @@ hipError_t launch_wpanel_deep(int storage, const Problem &p, int cx, int levels,
         for (int j = 0; j <= nmid; ++j) {
             const T *src = (const T *)(j == 0 ? P : mid[j - 1]);
             T *dst = (T *)(j == nmid ? Q : mid[j]);
-            k_wpanel_edge<T><<<g.nedge, 64, 0, side>>>(p, g, 2u * j, src, dst);
+            k_wpanel_edge<T, 8><<<g.nedge, 64, 0, side>>>(p, g, 2u * j, src, dst);
             if (j < nmid) {
                 Ring r = ring;
                 r.t = 2 + 2 * (nmid - 1 - j);

@@ -64,11 +64,12 @@ struct WaveGrid {
     int nin, nedge;                // interior waves, perimeter waves
     int r0, r1, nbl;               // interior rows [r0, r1) in nbl bands of the
                                    // interior kernel's height
-    // thin frame (the deep sweeps): the interior rows reach as close to the
-    // walls as their halo allows, the frame keeps the few rows below r0 and
-    // from r1 up (strips of at most six rows, eight chunks per perimeter wave:
-    // nthin waves) and the edge chunks of rows [r0, r1) in 62-row bands
-    int thin, nthin;
+    // thin frame: the interior rows reach as close to the walls as their
+    // halo allows, the frame keeps the few rows below r0 and from r1 up
+    // (strips of at most six rows, gl lanes per chunk, 64 / gl chunks per
+    // perimeter wave: nthin waves) and the edge chunks of rows [r0, r1) in
+    // 62-row bands
+    int thin, nthin, gl;
 };
 
 // Rows [Y0-1, Y0+62] (lanes) and columns [X0, X1) owned by wave (band, ck);
