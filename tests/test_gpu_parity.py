@@ -228,6 +228,37 @@ def test_panel6_against_oracle(orc, geom):
             assert np.array_equal(po[2:400], pref[2:400]), what
 
 
+@pytest.mark.parametrize("variant,lx,ly", [
+    ("panel6", 300, 136),   # 124 interior rows: the six-step march, thin frame
+    ("panel6", 300, 135),   # one row fewer: runs as panel4 (127 rows)
+    ("panel4", 300, 132),   # 124 interior rows of the four-step march
+    ("panel4", 300, 131),   # runs as panel, thin two-row strips
+    ("panel", 300, 66),     # 62 interior rows: the two-step march, thin frame
+    ("panel", 300, 65),     # 61: every wave on the perimeter
+    ("panel", 40, 5),
+])
+def test_thin_frame_thresholds(orc, variant, lx, ly):
+    """The interior rows of the panel sweeps run from H + 2 to ly - H (H = 1,
+    3, 5 halo rows) and the frame keeps H + 1 rows along each wall, packed
+    several chunks per perimeter wave; lattices at and just below the row
+    counts where a sweep switches to its thin frame or falls back, bit for
+    bit against the oracle, at chunk widths that leave a partial first
+    chunk and at the default."""
+    ch = OChannel.geometry(lx, ly, "B")
+    for steps in (7, 13):
+        _, lref, tref, pref = orc.run(ch, steps)
+        for cxk in (0, 7, 16):
+            with L.Lattice(dev_channel(ch)) as lat:
+                lat.set_tile(cxk, 0)
+                lat.init_equilibrium()
+                lat.step(steps, variant)
+                lo, to, po = lat.download()
+            what = f"{variant} {lx}x{ly} steps {steps} chunk {cxk}"
+            assert_same(lo, lref, what)
+            assert_same(to, tref, what + " tmp")
+            assert np.array_equal(po[2:ly], pref[2:ly]), what
+
+
 def test_panel6_long_run_and_switches(orc):
     """200 steps of panel6 mixed with the other sweeps on a lattice whose
     last band of interior rows is partial, against the oracle; and a lattice
