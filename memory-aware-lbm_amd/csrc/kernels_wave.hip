@@ -21,8 +21,8 @@
 //
 // Waves whose rows or columns touch the channel perimeter run a separate,
 // general kernel (k_wpanel_edge: wall operators, outlet extrapolation,
-// unknown populations; for the deep sweeps also the thin strips along the
-// walls, eight chunks per wave); the interior kernel (k_wpanel_in) runs BGK /
+// unknown populations; the strips of a few rows along the walls take
+// several chunks per wave); the interior kernel (k_wpanel_in) runs BGK /
 // bounce-back only, branch-free.  The second step's outlet density is
 // extrapolated from the post-stream densities of columns lx-2 and lx-1
 // gathered by the same lane just before their second collision
