@@ -247,8 +247,20 @@ template <int N, typename F> __device__ __forceinline__ void static_for(F &&f) {
 
 // WPG waves per workgroup take consecutive bands and march in lockstep (one
 // barrier per column), so the workgroup's loads of a column arrive at the
-// memory together as one contiguous run per population.
-constexpr int WPG = 8;
+// memory together as one contiguous run per population: eight waves.  The
+// six-step FP64 march with two columns in flight (211 VGPRs, two waves per
+// SIMD) takes four: an eight-wave workgroup needs two waves on every SIMD of
+// its CU, so no perimeter wave (230 VGPRs) fits beside it, while beside a
+// four-wave workgroup one does (16384 x 4096: 129.1-129.6 GLUPS against
+// 116.2-116.9; the four-step sweep, more memory-bound, loses with four: 95
+// against 113).  With one column in flight (131 / 166 VGPRs, three waves per
+// SIMD) the FP64 four- and six-step marches take twelve, the whole CU (115.0
+// against 112.6 for the four-step sweep at 63 columns).
+template <typename S, int DEPTH, int LEVELS> constexpr int wpg_of() {
+    if constexpr (sizeof(S) == 8 && LEVELS == 6 && DEPTH == 2) return 4;
+    else if constexpr (sizeof(S) == 8 && LEVELS >= 4 && DEPTH == 1) return 12;
+    else return 8;
+}
 
 // Interior waves, LEVELS time steps per launch (2, or 4 / 6 for the deeper
 // sweeps).  A wave's 64 lanes are rows Y0-H .. Y0+63-H with H = LEVELS-1 halo
@@ -260,9 +272,9 @@ constexpr int WPG = 8;
 // DEPTH = columns whose loads are in flight (1 or 2); cx % DEPTH == 0.
 // dstep: index of the launch's first collision relative to the counter.
 template <typename S, int DEPTH, int LEVELS>
-__global__ __launch_bounds__(64 * WPG) void k_wpanel_in(Problem p, WaveGrid g, uint32_t dstep,
-                                                        const S *__restrict__ P,
-                                                        S *__restrict__ Q) {
+__global__ __launch_bounds__((64 * wpg_of<S, DEPTH, LEVELS>())) void k_wpanel_in(
+    Problem p, WaveGrid g, uint32_t dstep, const S *__restrict__ P, S *__restrict__ Q) {
+    constexpr int WPG = wpg_of<S, DEPTH, LEVELS>();
     using T = typename Calc<S>::T;
     constexpr int H = LEVELS - 1;
     constexpr int ROWS = 64 - 2 * H;
@@ -517,6 +529,14 @@ bool wpanel_deep_possible(const Layout &L, int cx, int storage, int levels) {
     return g.thin && g.c_hi - g.c_lo >= 2;
 }
 
+// the interior kernel over g's interior waves, in workgroups of its WPG
+template <typename S, int DEPTH, int LEVELS>
+static void launch_in(const Problem &p, const WaveGrid &g, uint32_t dstep, const S *P, S *Q,
+                      hipStream_t st) {
+    constexpr int WPG = wpg_of<S, DEPTH, LEVELS>();
+    k_wpanel_in<S, DEPTH, LEVELS><<<(g.nin + WPG - 1) / WPG, 64 * WPG, 0, st>>>(p, g, dstep, P, Q);
+}
+
 // Perimeter waves on `side`, interior waves on `st`, joined back into `st`:
 // the two kernels write disjoint nodes and run concurrently.
 hipError_t launch_wpanel(int storage, const Problem &p, const WaveGrid &g, const void *P, void *Q,
@@ -525,7 +545,6 @@ hipError_t launch_wpanel(int storage, const Problem &p, const WaveGrid &g, const
     // two columns of loads in flight when the chunk width allows (three or
     // four measured no faster, for any storage format)
     const int depth = g.cx % 2 == 0 ? 2 : 1;
-    const int blocks = (g.nin + WPG - 1) / WPG;
     hipError_t e = hipEventRecord(fork, st);
     if (e == hipSuccess) e = hipStreamWaitEvent(side, fork, 0);
     if (e != hipSuccess) return e;
@@ -538,8 +557,8 @@ hipError_t launch_wpanel(int storage, const Problem &p, const WaveGrid &g, const
         if (!g.nin) return;
         const S *src = (const S *)P;
         S *dst = (S *)Q;
-        if (depth == 2) k_wpanel_in<S, 2, 2><<<blocks, 64 * WPG, 0, st>>>(p, g, 0u, src, dst);
-        else k_wpanel_in<S, 1, 2><<<blocks, 64 * WPG, 0, st>>>(p, g, 0u, src, dst);
+        if (depth == 2) launch_in<S, 2, 2>(p, g, 0u, src, dst, st);
+        else launch_in<S, 1, 2>(p, g, 0u, src, dst, st);
     };
     if (storage == 0) go(double{});
     else if (storage == 1) go(float{});
@@ -644,15 +663,14 @@ hipError_t launch_wpanel_deep(int storage, const Problem &p, int cx, int levels,
                 k_ring2<T><<<(r.n + 255) / 256, 256, 0, side>>>(p, r, 2u * j, src, dst);
             }
         }
-        const int blocks = (g.nin + WPG - 1) / WPG;
         const T *src = (const T *)P;
         T *dst = (T *)Q;
         if (levels == 4) {
-            if (depth == 2) k_wpanel_in<T, 2, 4><<<blocks, 64 * WPG, 0, st>>>(p, g, 0u, src, dst);
-            else k_wpanel_in<T, 1, 4><<<blocks, 64 * WPG, 0, st>>>(p, g, 0u, src, dst);
+            if (depth == 2) launch_in<T, 2, 4>(p, g, 0u, src, dst, st);
+            else launch_in<T, 1, 4>(p, g, 0u, src, dst, st);
         } else {
-            if (depth == 2) k_wpanel_in<T, 2, 6><<<blocks, 64 * WPG, 0, st>>>(p, g, 0u, src, dst);
-            else k_wpanel_in<T, 1, 6><<<blocks, 64 * WPG, 0, st>>>(p, g, 0u, src, dst);
+            if (depth == 2) launch_in<T, 2, 6>(p, g, 0u, src, dst, st);
+            else launch_in<T, 1, 6>(p, g, 0u, src, dst, st);
         }
     };
     if (storage == 0) go(double{});
