@@ -109,19 +109,19 @@ class SolveServer:
     # size and format: SWEEPS lists (minimum cells, variant) in increasing
     # size.  Measured on MI355X, one solve alone, graphs of 50 steps, GLUPS
     # fused / tight_block / panel / panel4 / panel6:
-    #   FP64 512 x 128 11.0 / 9.5 / 4.2 / 4.1 / 4.1, 1024 x 256 22.5 / 23.6 /
-    #   15.1 / 13.4 / 14.2, 1536 x 384 24.7 / 34.5 / 32.4 / 24.6 / 24.7,
-    #   2048 x 512 32.1 / 41.3 / 44.4 / 37.9 / 36.8, 3072 x 768 36.1 / 49.7 /
-    #   51.0 / 54.8 / 59.6, 4096 x 1024 37.2 / 53.9 / 52.5 / 67.9 / 73.1,
-    #   8192 x 2048 36.8 / 52.8 / 57.4 / 99.1 / 105.3;
-    #   FP32 (no panel6) 1024 x 256 29.5 / 16.6 / 16.0 / 10.8, 2048 x 512
-    #   48.5 / 41.5 / 45.8 / 36.7, 3072 x 768 61.2 / 59.7 / 73.1 / 63.2,
-    #   4096 x 1024 65.8 / 72.3 / 83.4 / 78.1, 8192 x 2048 66.6 / 76.5 /
-    #   107.2 / 120.3.
+    #   FP64 512 x 128 10.9 / 9.6 / 4.7 / 4.7 / 4.7, 1024 x 256 22.5 / 23.5 /
+    #   17.4 / 13.5 / 13.7, 1536 x 384 24.8 / 34.6 / 29.7 / 25.1 / 26.2,
+    #   2048 x 512 32.1 / 41.2 / 44.3 / 38.3 / 39.7, 3072 x 768 35.8 / 49.5 /
+    #   48.6 / 55.1 / 62.5, 4096 x 1024 37.1 / 53.8 / 49.8 / 68.3 / 78.0,
+    #   8192 x 2048 36.8 / 52.6 / 59.3 / 98.6 / 111.4;
+    #   FP32 (no panel6) 1024 x 256 29.2 / 16.2 / 15.8 / 10.8, 1536 x 384
+    #   34.3 / 31.1 / 33.9 / 24.4, 2048 x 512 47.8 / 41.0 / 50.8 / 37.0,
+    #   3072 x 768 61.1 / 59.4 / 75.9 / 63.0, 4096 x 1024 65.7 / 70.9 / 87.0 /
+    #   78.4, 8192 x 2048 66.1 / 76.3 / 105.1 / 119.1.
     SWEEPS = {"f64": ((0, "combine"), (400_000, "tight_block"), (1_000_000, "panel"),
                       (2_000_000, "panel6")),
-              "f32": ((0, "combine"), (1_500_000, "panel"), (8_000_000, "panel4")),
-              "f16": ((0, "combine"), (1_500_000, "panel"), (8_000_000, "panel4"))}
+              "f32": ((0, "combine"), (800_000, "panel"), (8_000_000, "panel4")),
+              "f16": ((0, "combine"), (800_000, "panel"), (8_000_000, "panel4"))}
 
     def __init__(self, hbm_budget: Optional[int] = None, max_active: int = 8,
                  slice_ms: float = 20.0, graph_steps: int = 50,
