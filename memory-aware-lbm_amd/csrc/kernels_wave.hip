@@ -252,8 +252,8 @@ template <int N, typename F> __device__ __forceinline__ void static_for(F &&f) {
 // SIMD) takes four: an eight-wave workgroup needs two waves on every SIMD of
 // its CU, so no perimeter wave (230 VGPRs) fits beside it, while beside a
 // four-wave workgroup one does (16384 x 4096: 129.1-129.6 GLUPS against
-// 116.2-116.9; the four-step sweep, more memory-bound, loses with four: 95
-// against 113).  With one column in flight (131 / 166 VGPRs, three waves per
+// 116.2-116.9, and 115.0 / 119.2 with two / one; the four-step sweep, more
+// memory-bound, loses with four: 95 against 113).  With one column in flight (131 / 166 VGPRs, three waves per
 // SIMD) the FP64 four- and six-step marches take twelve, the whole CU (115.0
 // against 112.6 for the four-step sweep at 63 columns); so do the FP64
 // two-step march (63.5 against 61.6) and the FP16 four-step one (239.0
