@@ -257,10 +257,12 @@ template <int N, typename F> __device__ __forceinline__ void static_for(F &&f) {
 // SIMD) the FP64 four- and six-step marches take twelve, the whole CU (115.0
 // against 112.6 for the four-step sweep at 63 columns); so do the FP64
 // two-step march (63.5 against 61.6) and the FP16 four-step one (239.0
-// against 228.1).  Measured and left at eight: FP32 (panel 119.9 / 120.1,
-// panel4 129.2 / 138.5 with twelve / eight), FP16 panel (212.7 / 211.7).
+// against 228.1).  The FP32 six-step march also takes four (134.7 against
+// 123.3; FP16 loses with four: 176.8 against 225.8).  Measured and left at
+// eight: FP32 (panel 119.9 / 120.1, panel4 129.2 / 138.5 with twelve /
+// eight), FP16 panel (212.7 / 211.7).
 template <typename S, int DEPTH, int LEVELS> constexpr int wpg_of() {
-    if constexpr (sizeof(S) == 8 && LEVELS == 6 && DEPTH == 2) return 4;
+    if constexpr (sizeof(S) >= 4 && LEVELS == 6 && DEPTH == 2) return 4;
     else if constexpr (sizeof(S) == 8 && LEVELS >= 4 && DEPTH == 1) return 12;
     else if constexpr (sizeof(S) == 8 && LEVELS == 2 && DEPTH == 2) return 12;
     else if constexpr (sizeof(S) == 2 && LEVELS == 4 && DEPTH == 2) return 12;
