@@ -3,35 +3,49 @@
 Workload (N=1): the 16384x4096 cylinder channel -- disc (lx/5, ly/2, ly/10),
 Re 100, u_max 0.02, regularized walls, Poiseuille inlet, zero-gradient
 pressure outlet -- with FP64 populations, bit-identical to the reference C
-code.  W warm-up steps, then K timed steps; one JSON line on stdout.
+code.  Every kernel's code is loaded first on a small channel; then, on the
+full lattice, the headline sweep first runs `--settle` steps (121: about
+70 ms, so the GPU's clocks and translation caches are warm), the lattice is
+re-initialised to the initial condition, then W untimed warm-up steps and K
+timed steps (HIP events on the lattice's stream around exactly those K
+steps); one JSON line on stdout.
 
 `value` is the GLUPS of the headline sweep variant (default panel6: six time
 steps per launch, five of them held in wave64 registers; the fastest FP64
-sweep on this lattice).  The other variants are timed on the same
-lattice and reported under "variants".  The populations are initialised on
+sweep on this lattice): 16384 x 4096 x K / the timed region.  K steps of a
+deep sweep are as many six-step launches as fit, then the shallower launches
+of the remainder; `steps_by_depth` says how many steps ran at each depth.
+The other variants are timed on the same lattice ("variants"), with the FP32
+and FP16 storage forms ("storage_forms").  The populations are initialised on
 the device, so everything is resident in HBM when the timed region starts.
 
-roofline.achieved follows the per-update figure of the hot path: 144 B (9
-populations read + 9 written, FP64) per lattice update, times the updates one
-launch of the dominant kernel performs (two per node for the two-step
-kernels), divided by that kernel's average launch duration measured with HIP
-events on the stream it runs on.  roofline.traffic is the HBM bytes per
-launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE (profiles/).  A
-two-step kernel moves fewer bytes than the updates it performs, which is how
-its achieved figure can pass the HBM peak.
+roofline: the dominant kernel is the one that carries the sweep's lattice
+updates (for panel6 the interior march k_wpanel_in<double, 2, 6>, whose
+perimeter kernels run beside it on a second stream).  Its average launch
+duration is measured live with HIP events on its own stream over the timed
+region.  achieved = the algorithmic bytes of one launch -- 144 B per node
+(nine FP64 populations read once and written once; 72 B FP32, 36 B FP16
+storage) times the nodes that kernel updates per launch -- divided by that
+duration; frac = achieved / 8 TB/s.  A k-step launch performs k updates per
+node for those bytes: the update-equivalent rate (achieved x k) is reported
+beside it under its own name and is not a roofline figure.  traffic is the
+same kernel's measured HBM bytes per launch, from rocprofv3 FETCH_SIZE and
+WRITE_SIZE (profiles/kernels_16384x4096.md); traffic_launch adds the
+perimeter and ring dispatches of the same launch.
 
-cpu_baseline: the reference's own OpenMP drivers (unsteady_origin_openmp and
-unsteady_tight_block_openmp, built from the reference sources into
-oracle/_ref/) timed on the host cores at the same lattice size for a bounded
-number of steps.
+cpu_baseline: the reference's own OpenMP sweeps (origin_openmp and
+tight_block_openmp from its lb.c / boundaries.c, built unmodified into
+oracle/_ref/libref_harness.so) on the same 16384x4096 disc-B channel, timed on
+the host cores of this machine for a bounded sample (10 warm-up + 100 timed
+steps by default, the GPU's 1:10 ratio), with the CPU model and core counts.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import re
-import subprocess
 import sys
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -39,17 +53,14 @@ sys.path.insert(0, os.path.join(ROOT, "memory-aware-lbm_amd"))
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "GLUPS & % of HBM roofline (FP64 vs FP32-store); serving p50/p99 latency"
-KERNEL_OF = {"origin": "k_collide<false>", "combine": "k_collide<true>",
-             "tight": "k_collide<true>", "tight_block": "k_block<double, 64>",
-             "panel": "k_wpanel_in<double, 2, 2>", "panel4": "k_wpanel_in<double, 1, 4>",
-             "panel6": "k_wpanel_in<double, 2, 6>"}
-LOWP_KERNEL_OF = {(s, v): k.replace("double", {"f32": "float", "f16": "_Float16"}[s])
-                  for s in ("f32", "f16") for v, k in
-                  (("combine", "k_collide_lp<double, true>"), ("tight_block", "k_block<double, 64>"),
-                   ("panel", "k_wpanel_in<double, 2, 2>"), ("panel4", "k_wpanel_in<double, 2, 4>"))}
-UPDATES_PER_LAUNCH = {"origin": 0.5, "combine": 1, "tight": 1, "tight_block": 2, "panel": 2,
-                      "panel4": 4, "panel6": 6}
+BYTES_PER_UPDATE = {"f64": 144.0, "f32": 72.0, "f16": 36.0}
 PROFILE = os.path.join(ROOT, "profiles", "kernels_16384x4096.md")
+# profile entry holding one whole launch (interior, perimeter and ring
+# dispatches added up) of the panel sweeps
+LAUNCH_KEY = {("panel", "f64"): "panel_launch<double>", ("panel4", "f64"): "panel4_launch<double>",
+              ("panel6", "f64"): "panel6_launch<double>", ("panel", "f32"): "panel_launch<float>",
+              ("panel4", "f32"): "panel4_launch<float>", ("panel", "f16"): "panel_launch<_Float16>",
+              ("panel4", "f16"): "panel4_launch<_Float16>"}
 
 
 def parse():
@@ -63,9 +74,12 @@ def parse():
     ap.add_argument("--variants", default="origin,combine,tight_block,panel,panel4",
                     help="variants timed besides the headline one (comma list, or '')")
     ap.add_argument("--other-steps", type=int, default=1000)
+    ap.add_argument("--settle", type=int, default=121,
+                    help="steps the headline sweep runs on the lattice before it is "
+                         "re-initialised for the warm-up and timed steps (0: none)")
     ap.add_argument("--storages", default="f32,f16",
                     help="reduced-precision storage forms, timed with the fused, the "
-                         "tight_block and the panel sweeps")
+                         "tight_block, the panel and the four-step sweeps")
     ap.add_argument("--batch", type=int, default=512,
                     help="channels of the batching workload (0 = skip)")
     ap.add_argument("--batch-seed", type=int, default=0)
@@ -77,35 +91,68 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrivals per second")
     ap.add_argument("--serve-seed", type=int, default=0)
     ap.add_argument("--cpu-baseline", default="auto", choices=["auto", "off"])
-    ap.add_argument("--cpu-steps", type=int, default=150,
-                    help="steps of each reference driver (150: about 10-15 s of CPU "
-                         "work each on 16 cores at 16384x4096)")
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-steps", type=int, default=100,
+                    help="timed steps of each reference OpenMP sweep")
+    ap.add_argument("--cpu-warmup", type=int, default=10)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="OpenMP threads (0: OMP_NUM_THREADS if set, else every physical "
+                         "core this process may run on)")
     return ap.parse_args()
 
 
-def time_variant(lbm, ch, variant, steps, warmup, storage="f64"):
+def warm_code(lbm):
+    """Load every sweep kernel's code object before any timed region: each
+    variant once on a 700x400 channel, through every launch depth it uses
+    (1 + 6 + 4 + 2 + 1 steps: the first collision, a six-, a four- and a
+    two-step launch, a single-step remainder), in every storage form."""
+    ch = lbm.Channel.geometry(700, 400, "B")
+    for storage in ("f64", "f32", "f16"):
+        with lbm.Lattice(ch, storage=storage) as lat:
+            for v in ("origin", "combine", "tight_block", "panel", "panel4", "panel6"):
+                lat.init_equilibrium()
+                for n in (7, 4, 2, 1):
+                    lat.step(n, v)
+            lat.sync()
+
+
+def dominant(recs):
+    """The record of the kernel carrying the most updates per launch."""
+    return max(recs, key=lambda r: (r["depth"], r["launches"] * r["depth"]))
+
+
+def time_variant(lbm, ch, variant, steps, warmup, storage="f64", settle=0):
     with lbm.Lattice(ch, storage=storage) as lat:
+        if settle:
+            # run the sweep on this lattice first so the GPU's clocks, its
+            # translation caches and the sweep's further buffers are warm,
+            # then start over from the initial condition
+            lat.init_equilibrium()
+            lat.step(settle, variant)
         lat.init_equilibrium()
         lat.step(max(warmup, 1), variant)   # leaves the state in the variant's convention
         lat.sync()
+        lat.kernel_timing(True)
         ms = lat.time_steps(steps, variant)
-        bpn = lat.bytes_per_node(variant)
+        recs = lat.kernel_times()
+        lat.kernel_timing(False)
     cells = ch.lx * ch.ly
-    glups = cells * steps / (ms * 1e-3) / 1e9
-    launches = steps / UPDATES_PER_LAUNCH[variant]
-    return dict(ms=ms, ms_per_step=ms / steps, glups=glups, design_bytes_per_node_step=bpn,
-                launch_ms=ms / launches)
+    dom = dominant(recs)
+    bpu = BYTES_PER_UPDATE[storage]
+    achieved = bpu * dom["nodes"] / (dom["avg_ms"] * 1e-3) / 1e9
+    by_depth = {}
+    for r in recs:
+        by_depth[str(r["depth"])] = by_depth.get(str(r["depth"]), 0) + r["depth"] * r["launches"]
+    return dict(ms=ms, ms_per_step=ms / steps, glups=cells * steps / (ms * 1e-3) / 1e9,
+                kernel=dom["kernel"], depth=dom["depth"], kernel_avg_ms=dom["avg_ms"],
+                kernel_nodes=dom["nodes"], achieved=achieved, steps_by_depth=by_depth,
+                launches=[{"kernel": r["kernel"], "depth": r["depth"], "launches": r["launches"],
+                           "avg_ms": round(r["avg_ms"], 5), "min_ms": round(r["min_ms"], 5),
+                           "max_ms": round(r["max_ms"], 5)} for r in recs])
 
 
-# entry of the profile's traffic line holding one launch of a variant (the
-# panel sweeps' interior and perimeter dispatches added up)
-TRAFFIC_KEY = {"panel": "panel_launch<double>", "panel4": "panel4_launch<double>",
-               "panel6": "panel6_launch<double>"}
-
-
-def measured_traffic(variant):
-    """HBM bytes per launch from the committed rocprofv3 PMC summary."""
+def measured_traffic(key):
+    """HBM bytes per launch of a kernel (or of a whole panel launch) from the
+    committed rocprofv3 PMC summary, or None."""
     try:
         text = open(PROFILE).read()
     except OSError:
@@ -115,7 +162,18 @@ def measured_traffic(variant):
         return None
     # names hold ", " themselves (template arguments): split at "=<digits>"
     table = {k.strip(): float(v) for k, v in re.findall(r"([^=]+)=(\d+)(?:,\s*)?", m.group(1))}
-    return table.get(TRAFFIC_KEY.get(variant, KERNEL_OF[variant]))
+    return table.get(key)
+
+
+def summary(r, storage):
+    out = {"glups": round(r["glups"], 4), "ms_per_step": round(r["ms_per_step"], 5),
+           "kernel": r["kernel"], "steps_per_launch": r["depth"],
+           "kernel_avg_ms": round(r["kernel_avg_ms"], 5),
+           "achieved_gbs": round(r["achieved"], 1),
+           "frac": round(r["achieved"] / HBM_PEAK_GBS, 4),
+           "update_equivalent_gbs": round(r["achieved"] * r["depth"], 1),
+           "steps_by_depth": r["steps_by_depth"]}
+    return out
 
 
 def batching(args):
@@ -138,10 +196,13 @@ def batching(args):
             b.step(10)
             ms = b.time_steps(args.batch_steps)
             glups = b.cells * args.batch_steps / (ms * 1e-3) / 1e9
-            bpu = {"f64": 144.0, "f32": 72.0, "f16": 36.0}[storage]
+            bpu = BYTES_PER_UPDATE[storage]
+            # a two-step launch reads and writes every population once
+            ach = glups * bpu / tune["depth"]
             out[storage] = {"glups": round(glups, 4), "ms_per_step": round(ms / args.batch_steps, 5),
-                            "achieved_gbs": round(glups * bpu, 1),
-                            "frac": round(glups * bpu / HBM_PEAK_GBS, 4),
+                            "achieved_gbs": round(ach, 1),
+                            "frac": round(ach / HBM_PEAK_GBS, 4),
+                            "update_equivalent_gbs": round(glups * bpu, 1),
                             "launch": {"depth": tune["depth"], "tile_cols": tune["tile_cols"],
                                        "ty_cap": tune["ty_cap"], "wgs_per_cu": tune["wgs_per_cu"],
                                        "graph_steps": tune["graph_steps"]},
@@ -187,38 +248,74 @@ def serving(args):
     return out
 
 
+def cpu_info():
+    """CPU model, physical cores and logical CPUs of the host, and how many of
+    them this process may run on (its affinity mask)."""
+    model, cores, logical = None, {}, 0
+    proc = phys = core = None
+    try:
+        text = open("/proc/cpuinfo").read()
+    except OSError:
+        text = ""
+    for block in text.strip().split("\n\n"):
+        kv = {}
+        for line in block.splitlines():
+            if ":" in line:
+                k, v = line.split(":", 1)
+                kv[k.strip()] = v.strip()
+        if "processor" not in kv:
+            continue
+        logical += 1
+        model = model or kv.get("model name")
+        proc, phys, core = int(kv["processor"]), kv.get("physical id", "0"), kv.get("core id", kv["processor"])
+        cores[proc] = (phys, core)
+    aff = os.sched_getaffinity(0) if hasattr(os, "sched_getaffinity") else set(cores)
+    return {"model": model, "logical_cpus": logical or os.cpu_count(),
+            "physical_cores": len(set(cores.values())) or None,
+            "cpus_in_affinity": len(aff),
+            "physical_cores_in_affinity": len({cores[p] for p in aff if p in cores}) or None}
+
+
 def cpu_baseline(args):
-    """Reference OpenMP drivers on the host cores, bounded sample."""
-    ref = os.path.join(ROOT, "oracle", "_ref")
-    threads = max(1, args.cpu_threads)
-    tb = max(64, (args.lx // threads) // 64 * 64)
-    env = dict(os.environ, STOP=str(args.cpu_steps), OMP_NUM_THREADS=str(threads),
-               OMP_PROC_BIND="close", OMP_PLACES="cores")
+    """The reference's OpenMP sweeps on the host cores, bounded sample."""
+    so = os.path.join(ROOT, "oracle", "_ref", "libref_harness.so")
+    if not os.path.exists(so):
+        return None
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    info = cpu_info()
+    threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS") or 0) \
+        or info["physical_cores_in_affinity"] or 1
+    # OpenMP reads these when the runtime loads with the harness below
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    lib = C.CDLL(so)
+    lib.ref_time.argtypes = [C.POINTER(ol._Cfg), C.c_int, C.POINTER(C.c_double)]
+    lib.ref_time.restype = C.c_int
+    ch = ol.Channel.geometry(args.lx, args.ly, "B")
+    tb = max(64, (args.lx // threads) // 64 * 64)   # thread_block: a multiple of blk_size
     res = {}
-    for name in ("origin_openmp", "tight_block_openmp"):
-        exe = os.path.join(ref, f"unsteady_{name}")
-        if not os.access(exe, os.X_OK):
+    for name, variant in (("origin_openmp", 0), ("tight_block_openmp", 3)):
+        cfg = ol._Cfg(ch.lx, ch.ly, ch.obst_x, ch.obst_y, ch.obst_r, ch.u_max, ch.reynolds,
+                      0, variant, 1, threads, 64, tb, args.cpu_steps, 1)
+        secs = C.c_double()
+        ran = lib.ref_time(C.byref(cfg), args.cpu_warmup, C.byref(secs))
+        if ran <= 0 or secs.value <= 0:
             continue
-        try:
-            out = subprocess.run([exe, str(args.lx), str(args.ly), "64", str(tb)], env=env,
-                                 capture_output=True, text=True, timeout=300).stdout
-        except subprocess.TimeoutExpired:
-            continue
-        m = re.search(r"time eclapsed in (\d+) steps, t_total=([0-9.]+)", out)
-        if not m:
-            continue
-        steps, t = int(m.group(1)), float(m.group(2))
-        # two-step drivers run STOP rounded up to even
-        ran = steps + (steps % 2 if "tight" in name else 0)
-        res[name] = dict(glups=args.lx * args.ly * ran / t / 1e9, t_total_s=t, steps=ran)
+        res[name] = dict(glups=args.lx * args.ly * ran / secs.value / 1e9, t_total_s=secs.value,
+                         steps=ran)
     if not res:
         return None
     best = max(res, key=lambda k: res[k]["glups"])
     return {"value": round(res[best]["glups"], 5), "unit": "GLUPS", "cores": threads,
             "kind": "reference",
-            "sample": f"reference {best} driver, {args.lx}x{args.ly}, "
-                      f"{res[best]['steps']} steps, blk 64, thread_block {tb}, "
-                      f"disc (lx/4, ly/2, ly/10+1) as the driver hard-codes it",
+            "sample": f"reference {best} sweep (lb.c, boundaries.c built unmodified), "
+                      f"{args.lx}x{args.ly}, disc (lx/5, ly/2, ly/10), Re 100, "
+                      f"{args.cpu_warmup} warm-up + {res[best]['steps']} timed steps, "
+                      f"blk 64, thread_block {tb}, {threads} OpenMP threads "
+                      f"(OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')}, "
+                      f"OMP_PLACES={os.environ.get('OMP_PLACES')})",
+            "cpu": info,
             "drivers": {k: {kk: round(vv, 5) for kk, vv in v.items()} for k, v in res.items()}}
 
 
@@ -230,35 +327,30 @@ def main():
         sys.exit(2)
     import lbm_amd as lbm
 
+    warm_code(lbm)
     ch = lbm.Channel.geometry(args.lx, args.ly, "B")
-    head = time_variant(lbm, ch, args.variant, args.steps, args.warmup)
-    cells = ch.lx * ch.ly
+    head = time_variant(lbm, ch, args.variant, args.steps, args.warmup, settle=args.settle)
     others = {}
     for v in [v for v in args.variants.split(",") if v and v != args.variant]:
-        r = time_variant(lbm, ch, v, args.other_steps, min(args.warmup, 20))
-        ach = 144.0 * cells * UPDATES_PER_LAUNCH[v] / (r["launch_ms"] * 1e-3) / 1e9
-        others[v] = {"glups": round(r["glups"], 4), "ms_per_step": round(r["ms_per_step"], 5),
-                     "kernel": KERNEL_OF[v], "launch_ms": round(r["launch_ms"], 5),
-                     "achieved_gbs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4)}
+        others[v] = summary(time_variant(lbm, ch, v, args.other_steps, min(args.warmup, 20)), "f64")
     storage = {}
     for sname in [v for v in args.storages.split(",") if v]:
-        bpu = {"f32": 72.0, "f16": 36.0}[sname]
-        storage[sname] = {"bytes_per_update": bpu}
+        storage[sname] = {"bytes_per_update": BYTES_PER_UPDATE[sname]}
         for v in ("combine", "tight_block", "panel", "panel4"):
             r = time_variant(lbm, ch, v, args.other_steps, min(args.warmup, 20), sname)
-            ach = bpu * cells * UPDATES_PER_LAUNCH[v] / (r["launch_ms"] * 1e-3) / 1e9
-            storage[sname][v] = {"glups": round(r["glups"], 4),
-                                 "ms_per_step": round(r["ms_per_step"], 5),
-                                 "kernel": LOWP_KERNEL_OF[(sname, v)],
-                                 "launch_ms": round(r["launch_ms"], 5),
-                                 "achieved_gbs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4)}
+            storage[sname][v] = summary(r, sname)
+            f64 = head if v == args.variant else None
+            base = (f64 or {}).get("glups") or others.get(v, {}).get("glups")
+            if base:
+                storage[sname][v]["vs_f64_same_sweep"] = round(r["glups"] / base, 3)
     batch = batching(args) if args.batch > 0 else None
     serve = serving(args) if args.serve > 0 else None
     copy_gbs = lbm.copy_bandwidth(2 << 30, 20)
-    kernel = KERNEL_OF[args.variant]
-    upl = UPDATES_PER_LAUNCH[args.variant]
-    achieved = 144.0 * cells * upl / (head["launch_ms"] * 1e-3) / 1e9
-    traffic = measured_traffic(args.variant) if (args.lx, args.ly) == (16384, 4096) else None
+    full = (args.lx, args.ly) == (16384, 4096)
+    traffic = measured_traffic(head["kernel"]) if full else None
+    launch_key = LAUNCH_KEY.get((args.variant, "f64"))
+    traffic_launch = measured_traffic(launch_key) if full and launch_key else None
+    avg_s = head["kernel_avg_ms"] * 1e-3
     out = {
         "metric": METRIC,
         "value": round(head["glups"], 4),
@@ -276,13 +368,23 @@ def main():
                                f"populations, {args.variant} sweep",
                    "lx": args.lx, "ly": args.ly, "variant": args.variant,
                    "disc": [ch.obst_x, ch.obst_y, ch.obst_r]},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "kernel": kernel,
-                     "launch_ms": round(head["launch_ms"], 5),
-                     "updates_per_node_per_launch": upl,
-                     "hbm_frac_measured": (round(traffic / (head["launch_ms"] * 1e-3) / 1e9
-                                                 / HBM_PEAK_GBS, 4) if traffic else None)},
+        "steps_by_depth": head["steps_by_depth"],
+        "launches": head["launches"],
+        "roofline": {"bound": "hbm", "achieved": round(head["achieved"], 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(head["achieved"] / HBM_PEAK_GBS, 4),
+                     "traffic": traffic,
+                     "kernel": head["kernel"],
+                     "kernel_avg_ms": round(head["kernel_avg_ms"], 5),
+                     "kernel_launches": next(r["launches"] for r in head["launches"]
+                                             if r["kernel"] == head["kernel"]
+                                             and r["depth"] == head["depth"]),
+                     "nodes_per_launch": int(head["kernel_nodes"]),
+                     "algorithmic_bytes_per_launch": int(144 * head["kernel_nodes"]),
+                     "steps_per_launch": head["depth"],
+                     "update_equivalent_gbs": round(head["achieved"] * head["depth"], 1),
+                     "hbm_frac_measured": (round(traffic / avg_s / 1e9 / HBM_PEAK_GBS, 4)
+                                           if traffic else None),
+                     "traffic_launch": traffic_launch},
         "variants": others,
         "storage_forms": storage,
         "batching": batch,

@@ -208,6 +208,32 @@ int lbm_hip_sync(lbm_ctx *ctx);
  * unsteady.c:387,558). */
 int lbm_hip_time_steps(lbm_ctx *ctx, int variant, int nsteps, double *ms_total);
 
+/* Launch timing of the kernels that carry a sweep's lattice updates
+ * (measurement support; no counterpart in the reference, whose driver times
+ * its whole loop with gettimeofday, unsteady.c:387,558).  After
+ * lbm_hip_kernel_timing(ctx, 1) every launch of such a kernel -- the interior
+ * march of PANEL / PANEL4 / PANEL6 (its perimeter kernels run beside it on a
+ * second stream and are not included), the tile kernel of TIGHT_BLOCK, the
+ * one-step collide kernel of every variant (ORIGIN, COMBINE / TIGHT, the
+ * first collision after init or upload and the single-step remainders) -- is
+ * bracketed by two HIP events on the stream that kernel runs on.  At most
+ * 4096 launches are recorded; launches inside graph captures are not.
+ * lbm_hip_kernel_timing(ctx, 0 or 1) also clears the record.
+ * lbm_hip_kernel_times waits for the context's work and writes one record
+ * per (kernel, steps per launch) in order of first launch: up to max into
+ * out, and the number of records into *count. */
+typedef struct {
+    char kernel[64];   /* kernel name as rocprofv3 prints it, e.g. "k_wpanel_in<double, 2, 6>" */
+    int depth;         /* time steps one launch advances: 1, 2, 4 or 6 */
+    int launches;      /* recorded launches */
+    double ms;         /* their summed durations, milliseconds */
+    double min_ms;     /* the shortest and the longest of them */
+    double max_ms;
+    double nodes;      /* lattice nodes one launch of the kernel updates */
+} lbm_kernel_record;
+int lbm_hip_kernel_timing(lbm_ctx *ctx, int on);
+int lbm_hip_kernel_times(lbm_ctx *ctx, lbm_kernel_record *out, int max, int *count);
+
 /* Device bytes one step of a variant moves by design (populations read plus
  * written), per interior node. */
 double lbm_hip_bytes_per_node(const lbm_ctx *ctx, int variant);

@@ -331,12 +331,10 @@ int lbm_hip_batch_create(const lbm_problem *probs, int n, lbm_batch **out) {
     if (storage < LBM_STORE_F64 || storage > LBM_STORE_F16)
         return fail(LBM_ERR_ARG, "unknown storage format");
     for (int c = 0; c < n; ++c) {
-        const lbm_problem &p = probs[c];
-        if (p.storage != storage) return fail(LBM_ERR_ARG, "mixed storage formats in one batch");
-        if (p.lx < 3 || p.ly < 3) return fail(LBM_ERR_ARG, "channel smaller than 3x3");
-        if (!(p.omega > 0.0 && p.omega <= 2.0)) return fail(LBM_ERR_ARG, "omega out of (0, 2]");
-        if (p.zgb && p.side[LBM_SIDE_RIGHT].kind != LBM_BC_PRESSURE)
-            return fail(LBM_ERR_ARG, "zgb needs a pressure operator on the right side");
+        if (probs[c].storage != storage)
+            return fail(LBM_ERR_ARG, "mixed storage formats in one batch");
+        const int rc = validate_problem(&probs[c]);
+        if (rc) return fail(rc, "channel " + std::to_string(c) + ": " + lbm_g_err);
     }
     lbm_batch *b = new (std::nothrow) lbm_batch();
     if (!b) return fail(LBM_ERR_NOMEM, "host allocation");

@@ -43,33 +43,19 @@ struct lbm_ctx {
     hipEvent_t fork, join;
     hipGraphExec_t graph[2];       // fused steps, by starting buffer
     hipGraphExec_t pgraph[2];      // panel launches, by starting buffer
+    // launch timing of the kernels that carry the lattice updates
+    int timing;                    // record them (lbm_hip_kernel_timing)
+    int capturing;                 // inside a graph capture: nothing is recorded
+    std::vector<hipEvent_t> tev;   // event pool, two per recorded launch
+    struct Timed { int depth; std::string name; double nodes; };
+    std::vector<Timed> tlog;
+    KernelProbe probe;
 };
 
 namespace {
 
 int side_len(const lbm_problem *p, int s) {
     return (s == LBM_SIDE_UPPER || s == LBM_SIDE_LOWER) ? p->lx + 2 : p->ly + 2;
-}
-
-int validate(const lbm_problem *p) {
-    if (!p) return fail(LBM_ERR_ARG, "null problem");
-    if (p->lx < 3 || p->ly < 3) return fail(LBM_ERR_ARG, "lx and ly must be >= 3");
-    if ((int64_t)(p->lx + 2) * (p->ly + 2) > (int64_t)1 << 31)
-        return fail(LBM_ERR_ARG, "lattice too large for 32-bit row indexing");
-    if (!(p->omega > 0.0 && p->omega <= 2.0))
-        return fail(LBM_ERR_ARG, "omega must lie in (0, 2]");
-    for (int s = 0; s < 4; ++s) {
-        const lbm_side_bc &b = p->side[s];
-        if (b.kind != LBM_BC_VELOCITY && b.kind != LBM_BC_PRESSURE)
-            return fail(LBM_ERR_ARG, "side kind must be velocity or pressure");
-        if (b.family != LBM_BC_REGULARIZED && b.family != LBM_BC_ZOUHE)
-            return fail(LBM_ERR_ARG, "side family must be regularized or Zou/He");
-    }
-    if (p->zgb && p->side[LBM_SIDE_RIGHT].kind != LBM_BC_PRESSURE)
-        return fail(LBM_ERR_ARG, "zgb needs a pressure operator on the right side");
-    if (p->storage != LBM_STORE_F64 && p->storage != LBM_STORE_F32 && p->storage != LBM_STORE_F16)
-        return fail(LBM_ERR_ARG, "unknown storage format");
-    return LBM_OK;
 }
 
 int upload_side(lbm_ctx *c, int s, const lbm_problem *p, const double *p0, const double *p1) {
@@ -87,12 +73,57 @@ int upload_side(lbm_ctx *c, int s, const lbm_problem *p, const double *p0, const
 
 double *dbuf(lbm_ctx *c, int k) { return static_cast<double *>(c->buf[k]); }
 
+const char *storage_name(int storage) {
+    return storage == LBM_STORE_F64 ? "double" : storage == LBM_STORE_F32 ? "float" : "_Float16";
+}
+
+// Launch timing: a probe for the next launch of a dominant kernel, or null
+// when timing is off, a graph is being captured or the record is full.
+constexpr size_t MAX_TIMED = 4096;
+
+KernelProbe *probe_for(lbm_ctx *c) {
+    if (!c->timing || c->capturing || c->tlog.size() >= MAX_TIMED) return nullptr;
+    const size_t k = 2 * c->tlog.size();
+    while (c->tev.size() < k + 2) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        c->tev.push_back(e);
+    }
+    c->probe = KernelProbe{};
+    c->probe.t0 = c->tev[k];
+    c->probe.t1 = c->tev[k + 1];
+    return &c->probe;
+}
+
+// keep the probe's launch if the launcher recorded one (a sweep without
+// interior waves launches no interior kernel)
+void probe_done(lbm_ctx *c, KernelProbe *pr, int depth) {
+    if (pr && pr->name[0]) c->tlog.push_back({depth, pr->name, pr->nodes});
+}
+
 // storage dispatch of the one-step kernels
 hipError_t run_collide(lbm_ctx *c, bool pull, int from, int to) {
-    if (c->storage == LBM_STORE_F64)
-        return launch_collide_f64(c->prob, pull, dbuf(c, from), dbuf(c, to), c->stream);
-    hipError_t e = launch_collide_lp(c->storage, c->prob, pull, c->buf[from], c->buf[to], c->stream);
-    return e == hipSuccess ? launch_bump(c->d_step, 1, c->stream) : e;
+    KernelProbe *pr = probe_for(c);
+    if (pr) {
+        if (c->storage == LBM_STORE_F64)
+            std::snprintf(pr->name, sizeof pr->name, "k_collide<%s>", pull ? "true" : "false");
+        else
+            std::snprintf(pr->name, sizeof pr->name, "k_collide_lp<%s, %s>",
+                          storage_name(c->storage), pull ? "true" : "false");
+        pr->nodes = (double)c->prob.lay.lx * c->prob.lay.ly;
+        hipError_t e0 = hipEventRecord(pr->t0, c->stream);
+        if (e0 != hipSuccess) return e0;
+    }
+    hipError_t e = c->storage == LBM_STORE_F64
+                       ? launch_collide_f64(c->prob, pull, dbuf(c, from), dbuf(c, to), c->stream)
+                       : launch_collide_lp(c->storage, c->prob, pull, c->buf[from], c->buf[to],
+                                           c->stream);
+    if (e == hipSuccess && pr) {
+        e = hipEventRecord(pr->t1, c->stream);
+        probe_done(c, pr, 1);
+    }
+    if (e == hipSuccess && c->storage != LBM_STORE_F64) e = launch_bump(c->d_step, 1, c->stream);
+    return e;
 }
 
 hipError_t run_stream(lbm_ctx *c, int from, int to) {
@@ -137,8 +168,10 @@ void drop_graphs(lbm_ctx *c) {
 // one two-step panel launch from buffer `from` into 1 - from
 hipError_t panel_launch(lbm_ctx *c, int from) {
     WaveGrid g = wave_grid(c->prob.lay, c->tile_x);
+    KernelProbe *pr = probe_for(c);
     hipError_t e = launch_wpanel(c->storage, c->prob, g, c->buf[from], c->buf[1 - from], c->stream,
-                                 c->side, c->fork, c->join);
+                                 c->side, c->fork, c->join, pr);
+    if (e == hipSuccess) probe_done(c, pr, 2);
     if (e == hipSuccess && c->storage != LBM_STORE_F64) e = launch_bump(c->d_step, 2, c->stream);
     return e;
 }
@@ -147,8 +180,10 @@ hipError_t panel_launch(lbm_ctx *c, int from) {
 // the perimeter frame passing through buf3 (and buf4)
 hipError_t deep_launch(lbm_ctx *c, int from, int levels) {
     void *mid[2] = {c->buf3, c->buf4};
+    KernelProbe *pr = probe_for(c);
     hipError_t e = launch_wpanel_deep(c->storage, c->prob, c->tile_x, levels, c->buf[from], mid,
-                                      c->buf[1 - from], c->stream, c->side, c->fork, c->join);
+                                      c->buf[1 - from], c->stream, c->side, c->fork, c->join, pr);
+    if (e == hipSuccess) probe_done(c, pr, levels);
     if (e == hipSuccess && c->storage != LBM_STORE_F64)
         e = launch_bump(c->d_step, (uint32_t)levels, c->stream);
     return e;
@@ -170,10 +205,12 @@ int capture_panel(lbm_ctx *c, int start) {
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
     hipError_t e = hipSuccess;
     int cur = start;
+    c->capturing = 1;
     for (int k = 0; k < c->graph_steps / 2 && e == hipSuccess; ++k) {
         e = panel_launch(c, cur);
         cur = 1 - cur;
     }
+    c->capturing = 0;
     hipError_t e2 = hipStreamEndCapture(c->stream, &g);
     if (e != hipSuccess) { if (g) (void)hipGraphDestroy(g); return hip_fail(e, "capture"); }
     if (e2 != hipSuccess) return hip_fail(e2, "hipStreamEndCapture");
@@ -189,10 +226,12 @@ int capture_fused(lbm_ctx *c, int start) {
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
     hipError_t e = hipSuccess;
     int cur = start;
+    c->capturing = 1;
     for (int k = 0; k < c->graph_steps && e == hipSuccess; ++k) {
         e = run_collide(c, true, cur, 1 - cur);
         cur = 1 - cur;
     }
+    c->capturing = 0;
     hipError_t e2 = hipStreamEndCapture(c->stream, &g);
     if (e != hipSuccess) { if (g) (void)hipGraphDestroy(g); return hip_fail(e, "capture"); }
     if (e2 != hipSuccess) return hip_fail(e2, "hipStreamEndCapture");
@@ -302,8 +341,19 @@ int step_impl(lbm_ctx *c, int variant, int nsteps) {
                 BlockGrid g = block_grid(c->prob.lay, c->tile_x,
                                          c->tile_y > 0 ? c->tile_y + 2 : 0,
                                          c->storage == LBM_STORE_F64 ? 8 : 4);
+                KernelProbe *pr = probe_for(c);
+                if (pr) {
+                    std::snprintf(pr->name, sizeof pr->name, "k_block<%s, %d>",
+                                  storage_name(c->storage), g.typ);
+                    pr->nodes = (double)c->prob.lay.lx * c->prob.lay.ly;
+                    HIPCHK(hipEventRecord(pr->t0, c->stream));
+                }
                 HIPCHK(launch_block(c->storage, c->prob, g, c->buf[c->cur], c->buf[1 - c->cur],
                                     c->stream));
+                if (pr) {
+                    HIPCHK(hipEventRecord(pr->t1, c->stream));
+                    probe_done(c, pr, 2);
+                }
                 if (c->storage != LBM_STORE_F64) HIPCHK(launch_bump(c->d_step, 2, c->stream));
             }
             c->cur = 1 - c->cur;
@@ -364,7 +414,7 @@ size_t lbm_hip_footprint_for(const lbm_problem *p, int variant) {
 int lbm_hip_create(const lbm_problem *p, lbm_ctx **out) {
     if (!out) return fail(LBM_ERR_ARG, "null output pointer");
     *out = nullptr;
-    int rc = validate(p);
+    int rc = validate_problem(p);
     if (rc) return rc;
     lbm_ctx *c = new (std::nothrow) lbm_ctx();
     if (!c) return fail(LBM_ERR_NOMEM, "host allocation");
@@ -453,6 +503,7 @@ int lbm_hip_destroy(lbm_ctx *c) {
     if (c->buf3) (void)hipFree(c->buf3);
     if (c->buf4) (void)hipFree(c->buf4);
     if (c->side) (void)hipStreamSynchronize(c->side);
+    for (hipEvent_t e : c->tev) (void)hipEventDestroy(e);
     if (c->fork) (void)hipEventDestroy(c->fork);
     if (c->join) (void)hipEventDestroy(c->join);
     if (c->side) (void)hipStreamDestroy(c->side);
@@ -581,7 +632,11 @@ int lbm_hip_device_memory(size_t *free_bytes, size_t *total_bytes) {
 
 int lbm_hip_set_tile(lbm_ctx *c, int tile_x, int tile_y) {
     if (!c || tile_x < 0 || tile_y < 0) return fail(LBM_ERR_ARG, "bad tile");
-    if (tile_x != c->tile_x || tile_y != c->tile_y) drop_graphs(c);
+    if (tile_x != c->tile_x || tile_y != c->tile_y) {
+        // a replay still in flight owns its executable graph
+        HIPCHK(hipStreamSynchronize(c->stream));
+        drop_graphs(c);
+    }
     c->tile_x = tile_x;
     c->tile_y = tile_y;
     return LBM_OK;
@@ -630,6 +685,43 @@ int lbm_hip_diagnostics(lbm_ctx *c, double out[4]) {
     }
     HIPCHK(hipMemcpyAsync(out, c->d_out4, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+int lbm_hip_kernel_timing(lbm_ctx *c, int on) {
+    if (!c) return fail(LBM_ERR_ARG, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->timing = on != 0;
+    c->tlog.clear();
+    return LBM_OK;
+}
+
+int lbm_hip_kernel_times(lbm_ctx *c, lbm_kernel_record *out, int max, int *count) {
+    if (!c || !count || max < 0 || (max > 0 && !out)) return fail(LBM_ERR_ARG, "bad argument");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::vector<lbm_kernel_record> recs;
+    for (size_t k = 0; k < c->tlog.size(); ++k) {
+        const lbm_ctx::Timed &t = c->tlog[k];
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, c->tev[2 * k], c->tev[2 * k + 1]));
+        size_t j = 0;
+        while (j < recs.size() && !(recs[j].depth == t.depth && t.name == recs[j].kernel)) ++j;
+        if (j == recs.size()) {
+            lbm_kernel_record r{};
+            std::snprintf(r.kernel, sizeof r.kernel, "%s", t.name.c_str());
+            r.depth = t.depth;
+            r.nodes = t.nodes;
+            r.min_ms = ms;
+            r.max_ms = ms;
+            recs.push_back(r);
+        }
+        recs[j].launches += 1;
+        recs[j].ms += ms;
+        recs[j].min_ms = ms < recs[j].min_ms ? ms : recs[j].min_ms;
+        recs[j].max_ms = ms > recs[j].max_ms ? ms : recs[j].max_ms;
+    }
+    *count = (int)recs.size();
+    for (int j = 0; j < max && j < (int)recs.size(); ++j) out[j] = recs[j];
     return LBM_OK;
 }
 

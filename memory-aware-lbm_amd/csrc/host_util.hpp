@@ -30,6 +30,29 @@ inline int hip_fail(hipError_t e, const char *what) {
 
 
 
+// Checks shared by single contexts and batches: sizes, relaxation, side
+// operators, outlet and storage.
+inline int validate_problem(const lbm_problem *p) {
+    if (!p) return fail(LBM_ERR_ARG, "null problem");
+    if (p->lx < 3 || p->ly < 3) return fail(LBM_ERR_ARG, "lx and ly must be >= 3");
+    if ((int64_t)(p->lx + 2) * (p->ly + 2) > (int64_t)1 << 31)
+        return fail(LBM_ERR_ARG, "lattice too large for 32-bit row indexing");
+    if (!(p->omega > 0.0 && p->omega <= 2.0))
+        return fail(LBM_ERR_ARG, "omega must lie in (0, 2]");
+    for (int s = 0; s < 4; ++s) {
+        const lbm_side_bc &b = p->side[s];
+        if (b.kind != LBM_BC_VELOCITY && b.kind != LBM_BC_PRESSURE)
+            return fail(LBM_ERR_ARG, "side kind must be velocity or pressure");
+        if (b.family != LBM_BC_REGULARIZED && b.family != LBM_BC_ZOUHE)
+            return fail(LBM_ERR_ARG, "side family must be regularized or Zou/He");
+    }
+    if (p->zgb && p->side[LBM_SIDE_RIGHT].kind != LBM_BC_PRESSURE)
+        return fail(LBM_ERR_ARG, "zgb needs a pressure operator on the right side");
+    if (p->storage != LBM_STORE_F64 && p->storage != LBM_STORE_F32 && p->storage != LBM_STORE_F16)
+        return fail(LBM_ERR_ARG, "unknown storage format");
+    return LBM_OK;
+}
+
 // host node layout <-> device planes, interior nodes only
 inline int transfer_f64(const lbm::Layout &L, double *host, double *devbuf, bool to_device) {
     size_t plane_bytes = (size_t)L.plane * sizeof(double);

@@ -36,6 +36,7 @@
 // FP64 collisions rounded stochastically; FP16: FP32 collisions; FP64 walls).
 // For reduced storage each intermediate step is rounded to S exactly as a
 // store would round it, so k steps here equal k single steps bit for bit.
+#include <cstdio>
 #include <utility>
 
 #include "launch.hpp"
@@ -536,18 +537,32 @@ bool wpanel_deep_possible(const Layout &L, int cx, int storage, int levels) {
     return g.thin && g.c_hi - g.c_lo >= 2;
 }
 
-// the interior kernel over g's interior waves, in workgroups of its WPG
+template <typename S> constexpr const char *type_name() {
+    return sizeof(S) == 8 ? "double" : sizeof(S) == 4 ? "float" : "_Float16";
+}
+
+// the interior kernel over g's interior waves, in workgroups of its WPG;
+// a probe brackets it with events on st
 template <typename S, int DEPTH, int LEVELS>
 static void launch_in(const Problem &p, const WaveGrid &g, uint32_t dstep, const S *P, S *Q,
-                      hipStream_t st) {
+                      hipStream_t st, KernelProbe *probe) {
     constexpr int WPG = wpg_of<S, DEPTH, LEVELS>();
+    if (probe) {
+        std::snprintf(probe->name, sizeof probe->name, "k_wpanel_in<%s, %d, %d>", type_name<S>(),
+                      DEPTH, LEVELS);
+        // rows [r0, r1) of the interior chunks [c_lo, c_hi)
+        probe->nodes = (double)(g.r1 - g.r0) * (double)(g.c_hi - g.c_lo) * g.cx;
+        (void)hipEventRecord(probe->t0, st);
+    }
     k_wpanel_in<S, DEPTH, LEVELS><<<(g.nin + WPG - 1) / WPG, 64 * WPG, 0, st>>>(p, g, dstep, P, Q);
+    if (probe) (void)hipEventRecord(probe->t1, st);
 }
 
 // Perimeter waves on `side`, interior waves on `st`, joined back into `st`:
 // the two kernels write disjoint nodes and run concurrently.
 hipError_t launch_wpanel(int storage, const Problem &p, const WaveGrid &g, const void *P, void *Q,
-                         hipStream_t st, hipStream_t side, hipEvent_t fork, hipEvent_t join) {
+                         hipStream_t st, hipStream_t side, hipEvent_t fork, hipEvent_t join,
+                         KernelProbe *probe) {
     if (g.cx < 3) return hipErrorInvalidValue;
     // two columns of loads in flight when the chunk width allows (three or
     // four measured no faster, for any storage format)
@@ -564,8 +579,8 @@ hipError_t launch_wpanel(int storage, const Problem &p, const WaveGrid &g, const
         if (!g.nin) return;
         const S *src = (const S *)P;
         S *dst = (S *)Q;
-        if (depth == 2) launch_in<S, 2, 2>(p, g, 0u, src, dst, st);
-        else launch_in<S, 1, 2>(p, g, 0u, src, dst, st);
+        if (depth == 2) launch_in<S, 2, 2>(p, g, 0u, src, dst, st, probe);
+        else launch_in<S, 1, 2>(p, g, 0u, src, dst, st, probe);
     };
     if (storage == 0) go(double{});
     else if (storage == 1) go(float{});
@@ -635,7 +650,7 @@ __global__ __launch_bounds__(256) void k_ring2(Problem p, Ring r, uint32_t dstep
 
 hipError_t launch_wpanel_deep(int storage, const Problem &p, int cx, int levels, const void *P,
                               void *const *mid, void *Q, hipStream_t st, hipStream_t side,
-                              hipEvent_t fork, hipEvent_t join) {
+                              hipEvent_t fork, hipEvent_t join, KernelProbe *probe) {
     if (!wpanel_deep_possible(p.lay, cx, storage, levels)) return hipErrorInvalidValue;
     const Layout &L = p.lay;
     const WaveGrid g = wave_grid_deep(L, cx, storage, levels);
@@ -673,11 +688,11 @@ hipError_t launch_wpanel_deep(int storage, const Problem &p, int cx, int levels,
         const T *src = (const T *)P;
         T *dst = (T *)Q;
         if (levels == 4) {
-            if (depth == 2) launch_in<T, 2, 4>(p, g, 0u, src, dst, st);
-            else launch_in<T, 1, 4>(p, g, 0u, src, dst, st);
+            if (depth == 2) launch_in<T, 2, 4>(p, g, 0u, src, dst, st, probe);
+            else launch_in<T, 1, 4>(p, g, 0u, src, dst, st, probe);
         } else {
-            if (depth == 2) launch_in<T, 2, 6>(p, g, 0u, src, dst, st);
-            else launch_in<T, 1, 6>(p, g, 0u, src, dst, st);
+            if (depth == 2) launch_in<T, 2, 6>(p, g, 0u, src, dst, st, probe);
+            else launch_in<T, 1, 6>(p, g, 0u, src, dst, st, probe);
         }
     };
     if (storage == 0) go(double{});

@@ -7,6 +7,17 @@
 
 namespace lbm {
 
+// Optional timing of the kernel that carries a launch's lattice updates
+// (measurement support).  A launcher given a probe records t0 and t1 on that
+// kernel's own stream right before and after launching it, and fills in the
+// kernel's name (as the profiler prints it) and the nodes it updates per
+// launch.
+struct KernelProbe {
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    char name[64] = {0};
+    double nodes = 0.;
+};
+
 // FP64 (bit-exact) path: kernels_f64.hip
 hipError_t launch_collide_f64(const Problem &p, bool pull, const double *src, double *dst,
                               hipStream_t st);
@@ -85,7 +96,8 @@ __host__ __device__ inline WaveSpan wave_span(const Layout &L, const WaveGrid &g
 
 WaveGrid wave_grid(const Layout &L, int cx);
 hipError_t launch_wpanel(int storage, const Problem &p, const WaveGrid &g, const void *P, void *Q,
-                         hipStream_t st, hipStream_t side, hipEvent_t fork, hipEvent_t join);
+                         hipStream_t st, hipStream_t side, hipEvent_t fork, hipEvent_t join,
+                         KernelProbe *probe = nullptr);
 // LEVELS = 4 or 6 steps per launch: the interior waves march LEVELS levels
 // in registers (P -> Q); the perimeter frame runs LEVELS/2 two-step launches
 // through LEVELS/2 - 1 further buffers mid[] (P -> mid[0] over the frame
@@ -94,6 +106,6 @@ hipError_t launch_wpanel(int storage, const Problem &p, const WaveGrid &g, const
 bool wpanel_deep_possible(const Layout &L, int cx, int storage, int levels);
 hipError_t launch_wpanel_deep(int storage, const Problem &p, int cx, int levels, const void *P,
                               void *const *mid, void *Q, hipStream_t st, hipStream_t side,
-                              hipEvent_t fork, hipEvent_t join);
+                              hipEvent_t fork, hipEvent_t join, KernelProbe *probe = nullptr);
 
 }  // namespace lbm

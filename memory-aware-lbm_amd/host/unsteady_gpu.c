@@ -9,7 +9,14 @@
  * with radius ly/10+1, Re 100, u_max 0.02, regularized walls, Poiseuille
  * inlet, pressure outlet with zero-gradient density), the same step loop and
  * the same progress and timing lines.  variant is one of origin, combine,
- * tight, tight_block, panel (default: $LBM_VARIANT, else origin).
+ * tight, tight_block, panel.  It is taken from argv[5] if
+ * given, else from the program's own name when that is one of the
+ * reference's executables, unsteady_<variant>[_openmp][_papi]
+ * (CMakeLists.txt:27-50; the Makefile installs those names as links to this
+ * program under bin/, so scripts such as run_exps.sh:67-76 run unchanged; the
+ * _openmp and _papi suffixes select the same device sweep), else from
+ * $LBM_VARIANT, else origin.  The _papi names print no counter lines: the
+ * counter report of a sweep is `python -m lbm_amd.counters`.
  *
  * Extra environment (none needed for the reference behaviour):
  *   LBM_DISC=B        disc (lx/5, ly/2, ly/10) instead of the driver's own
@@ -66,6 +73,28 @@ static double poiseuille(int y) {                       /* unsteady.c:155-159 */
     return 4. * uMax / (L * L) * (L * yy - yy * yy);
 }
 
+/* the sweep named by a reference executable name, unsteady_<variant>
+ * [_openmp][_papi] (CMakeLists.txt:27-50), or NULL */
+static const char *variant_of_name(const char *path) {
+    static const char *const names[] = {"origin", "combine", "tight_block", "tight", "panel"};
+    static char v[32];
+    const char *base = strrchr(path, '/');
+    base = base ? base + 1 : path;
+    if (strncmp(base, "unsteady_", 9)) return NULL;
+    base += 9;
+    for (size_t k = 0; k < sizeof names / sizeof names[0]; ++k) {
+        size_t n = strlen(names[k]);
+        const char *rest = base + n;
+        if (strncmp(base, names[k], n)) continue;
+        if (*rest && strcmp(rest, "_openmp") && strcmp(rest, "_papi") &&
+            strcmp(rest, "_openmp_papi"))
+            continue;
+        snprintf(v, sizeof v, "%s", names[k]);
+        return v;
+    }
+    return NULL;
+}
+
 int main(int argc, char *argv[]) {
     if (argc < 4) {
         fprintf(stderr, "usage: STOP=<steps> %s lx ly blk_size [thread_block] [variant]\n",
@@ -78,7 +107,8 @@ int main(int argc, char *argv[]) {
     ly = atoi(argv[2]);
     blk_size = atoi(argv[3]);
     thread_block = argc > 4 ? atoi(argv[4]) : 0;
-    const char *variant = argc > 5 ? argv[5] : getenv("LBM_VARIANT");
+    const char *variant = argc > 5 ? argv[5] : variant_of_name(argv[0]);
+    if (!variant) variant = getenv("LBM_VARIANT");
     if (!variant) variant = "origin";
     const char *disc = getenv("LBM_DISC");
     int zouhe = getenv("LBM_FAMILY") && !strcmp(getenv("LBM_FAMILY"), "zouhe");

@@ -43,6 +43,12 @@ class _SideBC(C.Structure):
     _fields_ = [("kind", C.c_int), ("family", C.c_int), ("p0", _P), ("p1", _P)]
 
 
+class _KernelRecord(C.Structure):
+    _fields_ = [("kernel", C.c_char * 64), ("depth", C.c_int), ("launches", C.c_int),
+                ("ms", C.c_double), ("min_ms", C.c_double), ("max_ms", C.c_double),
+                ("nodes", C.c_double)]
+
+
 class _Problem(C.Structure):
     _fields_ = [("lx", C.c_int), ("ly", C.c_int), ("obst_x", C.c_int),
                 ("obst_y", C.c_int), ("obst_r", C.c_int), ("omega", C.c_double),
@@ -86,6 +92,9 @@ def library():
         "lbm_hip_make_periodic": (C.c_int, [vp]),
         "lbm_hip_query": (C.c_int, [vp]),
         "lbm_hip_device_memory": (C.c_int, [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+        "lbm_hip_kernel_timing": (C.c_int, [vp, C.c_int]),
+        "lbm_hip_kernel_times": (C.c_int, [vp, C.POINTER(_KernelRecord), C.c_int,
+                                           C.POINTER(C.c_int)]),
         "lbm_hip_last_error": (C.c_char_p, []),
     }
     for name, (res, args) in sig.items():
@@ -259,6 +268,26 @@ class Lattice:
         _check(library().lbm_hip_time_steps(self._h, VARIANT[variant], int(n),
                                             C.byref(ms)), "time_steps")
         return ms.value
+
+    def kernel_timing(self, on: bool = True):
+        """Start (or stop) recording the launches of the kernels that carry
+        the lattice updates; clears the record."""
+        _check(library().lbm_hip_kernel_timing(self._h, int(bool(on))), "kernel_timing")
+
+    def kernel_times(self) -> list:
+        """One dict per (kernel, steps per launch) recorded since
+        kernel_timing(True): kernel, depth, launches, ms (summed), avg_ms,
+        min_ms, max_ms, nodes (updated per launch)."""
+        n = C.c_int()
+        _check(library().lbm_hip_kernel_times(self._h, None, 0, C.byref(n)), "kernel_times")
+        recs = (_KernelRecord * max(n.value, 1))()
+        _check(library().lbm_hip_kernel_times(self._h, recs, n.value, C.byref(n)), "kernel_times")
+        out = []
+        for r in recs[:n.value]:
+            out.append({"kernel": r.kernel.decode(), "depth": r.depth, "launches": r.launches,
+                        "ms": r.ms, "avg_ms": r.ms / r.launches if r.launches else 0.0,
+                        "min_ms": r.min_ms, "max_ms": r.max_ms, "nodes": r.nodes})
+        return out
 
     def bytes_per_node(self, variant: str) -> float:
         return float(library().lbm_hip_bytes_per_node(self._h, VARIANT[variant]))

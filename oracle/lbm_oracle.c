@@ -2,9 +2,12 @@
  * lbm_oracle.c -- CPU oracle for the D2Q9 BGK collide-stream path.
  *
  * TEST INFRASTRUCTURE ONLY (see lbm_oracle.h for who may use it and how
- * every function is pinned).  Serial, double precision, built with
- * -O2 -ffp-contract=off so that every operation rounds exactly as the
+ * every function is pinned).  Double precision, built with
+ * -O3 -ffp-contract=off so that every operation rounds exactly as the
  * reference's gcc -O3 x86-64 build does (SSE2, no fused multiply-add).
+ * The one-step sweeps spread their column loops over OpenMP threads, which
+ * changes no result (each node's work is independent); the two-step orders
+ * stay serial, as their visiting order is what they restate.
  *
  * State convention (the reference's): buffer L holds the post-stream,
  * pre-collision populations of the current step; buffer T holds what the
@@ -295,6 +298,7 @@ static int chan_init(chan *c, const orc_cfg *cfg) {
     int fam_pres = cfg->bc_family ? K_ZH_PRES : K_REG_PRES;
     for (int y = 2; y <= c->ly - 1; ++y) c->prho[y] = 1.;
     int ox = cfg->obst_x, oy = cfg->obst_y, r = cfg->obst_r;
+#pragma omp parallel for schedule(static)
     for (int x = 1; x <= c->lx; ++x)
         for (int y = 1; y <= c->ly; ++y) {
             double u = orc_poiseuille(y, c->ly, cfg->u_max);
@@ -334,15 +338,22 @@ static void zgb_update(chan *c) {
 /* origin: collide everything in place, then stream into T and swap
  * (lb.c:172-201, 246-287).  combine (lb.c:353-390) performs the same
  * per-node work fused; its result is identical, so it shares this path. */
+/* Both one-step orders run their column loop on OpenMP threads: a node's
+ * collision reads and writes that node alone, and every slot a push writes
+ * has exactly one source node, so the result does not depend on the thread
+ * count or schedule (bit for bit). */
 static void step_origin(chan *c) {
+#pragma omp parallel for schedule(static)
     for (int x = 1; x <= c->lx; ++x)
         for (int y = 1; y <= c->ly; ++y) apply(c, c->L, x, y);
+#pragma omp parallel for schedule(static)
     for (int x = 1; x <= c->lx; ++x)
         for (int y = 1; y <= c->ly; ++y) push(c, c->L, c->T, x, y);
     double *s = c->L; c->L = c->T; c->T = s;
 }
 
 static void step_combine(chan *c) {
+#pragma omp parallel for schedule(static)
     for (int x = 1; x <= c->lx; ++x)
         for (int y = 1; y <= c->ly; ++y) {
             apply(c, c->L, x, y);

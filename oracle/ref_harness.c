@@ -14,6 +14,7 @@
  */
 #include <stdlib.h>
 #include <string.h>
+#include <sys/time.h>
 #include "lb.h"
 #include "boundaries.h"
 
@@ -227,6 +228,45 @@ static int run_with(const ref_cfg *c, int periodic_after, double *lat_out, doubl
     if (tmp_out) copy_out(h_sim.tmpLattice, tmp_out);
     if (prho_out)
         for (int y = 0; y < h_ly + 2; ++y) prho_out[y] = pressureBoundary[y].rho;
+    return ran;
+}
+
+/* CPU baseline timing: the reference's sweep `c->variant` (with its OpenMP
+ * flavour when c->openmp) on the channel of `c`, `warmup` untimed steps,
+ * then c->steps timed steps in the driver's loop order (unsteady.c:424-531:
+ * ZGB, collision, stream), timed with gettimeofday as the driver does
+ * (unsteady.c:387,558, eval_tools.h).  *seconds receives the timed loop's
+ * wall time.  Returns the number of timed steps run (two-step sweeps round
+ * up to even), negative on a bad variant. */
+int ref_time(const ref_cfg *c, int warmup, double *seconds) {
+    void (*coll)(Simulation *) = NULL;
+    void (*strm)(Simulation *) = NULL;
+    int twostep = 0;
+    switch (c->variant) {
+    case 0: coll = c->openmp ? collide_openmp : collide;
+            strm = c->openmp ? propagate_openmp : propagate; break;
+    case 1: coll = c->openmp ? collide_with_stream_openmp : collide_with_stream;
+            strm = finalize_stream; break;
+    case 3: coll = c->openmp ? collide_tight_block_openmp : collide_tight_block;
+            twostep = 1; break;
+    default: return -1;
+    }
+    setup(c);
+#ifdef _OPENMP
+    omp_set_num_threads(NUM_THREADS);
+#endif
+    int ran = 0;
+    struct timeval t0, t1;
+    for (iT = 0; iT < warmup + c->steps; iT += twostep ? 2 : 1) {
+        if (iT == warmup || (iT > warmup && ran == 0)) gettimeofday(&t0, NULL);
+        if (c->zgb) updateZeroGradientBoundary();
+        coll(&h_sim);
+        if (!twostep) strm(&h_sim);
+        if (iT >= warmup) ran += twostep ? 2 : 1;
+    }
+    gettimeofday(&t1, NULL);
+    *seconds = (double)(t1.tv_sec - t0.tv_sec) + 1e-6 * (double)(t1.tv_usec - t0.tv_usec);
+    release();
     return ran;
 }
 

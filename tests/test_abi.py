@@ -149,3 +149,13 @@ def test_create_rejects_bad_problems_without_gpu(field, value, message):
     assert rc == -1                                        # LBM_ERR_ARG
     assert h.value is None
     assert message in lib.lbm_hip_last_error().decode()
+
+
+def test_reference_executable_names_installed():
+    """unsteady_<variant>[_openmp][_papi] for the reference's five sweeps
+    (CMakeLists.txt:27-50) resolve to the GPU driver."""
+    exe = os.path.realpath(os.path.join(PKG, "unsteady_gpu"))
+    for v in ("origin", "combine", "tight", "tight_block", "panel"):
+        for s in ("", "_openmp", "_papi", "_openmp_papi"):
+            link = os.path.join(PKG, "bin", f"unsteady_{v}{s}")
+            assert os.path.realpath(link) == exe, link

@@ -53,3 +53,25 @@ def test_driver_savevel_format(orc, tmp_path):
         y, x = divmod(k, lx)
         rho, ux, uy = orc.macros(lref[x + 1, y + 1])
         assert rows[k] == f"{ux:f},{1.0:f},{uy:f}"
+
+
+@pytest.mark.parametrize("name", ["unsteady_tight_block_openmp", "unsteady_panel",
+                                  "unsteady_combine_openmp_papi"])
+def test_reference_executable_names(orc, tmp_path, name):
+    """The reference's scripts call unsteady_<variant>[_openmp][_papi] lx ly
+    blk thread_block (run_exps.sh:67-76); the installed names run the named
+    sweep with no further argument, bit-identical to the oracle."""
+    lx, ly, steps = 96, 40, 30
+    exe = os.path.join(ROOT, "memory-aware-lbm_amd", "bin", name)
+    dump = str(tmp_path / "dump.bin")
+    env = dict(os.environ, STOP=str(steps), LBM_DUMP=dump)
+    env.pop("LBM_VARIANT", None)
+    r = subprocess.run([exe, str(lx), str(ly), "8", "16"], env=env, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    variant = name[len("unsteady_"):].replace("_openmp", "").replace("_papi", "")
+    assert f"use case {variant} " in r.stdout, r.stdout
+    raw = np.fromfile(dump, dtype=np.float64).reshape(2, lx + 2, ly + 2, 9)
+    _, lref, tref, _ = orc.run(Channel.geometry(lx, ly, "A"), steps)
+    assert np.array_equal(raw[0][1:-1, 1:-1], lref[1:-1, 1:-1])
+    assert np.array_equal(raw[1][1:-1, 1:-1], tref[1:-1, 1:-1])

@@ -68,9 +68,14 @@ def test_trajectory_golden(name, variant):
 
 
 @pytest.mark.parametrize("geom", ["A", "B"])
-@pytest.mark.parametrize("variant", ["origin", "combine", "tight_block", "panel"])
+@pytest.mark.parametrize("variant", ["origin", "combine", "tight", "tight_block", "panel",
+                                     "panel4", "panel6"])
 def test_c1_parity_2000_steps(orc, geom, variant):
-    """256x128, Re 100, 2000 steps: the parity configuration."""
+    """256x128, Re 100, 2000 steps: the parity configuration, every sweep.
+    The lattice's 126 interior rows are too few for the four- and six-step
+    marches (they need 124 rows inside an H-row halo on each side), so
+    panel4 and panel6 run as the two-step panel here, as documented;
+    test_deep_sweeps_2000_steps runs them deep for as long."""
     ch = OChannel.geometry(256, 128, geom)
     ran, lat_ref, tmp_ref, prho_ref = orc.run(ch, 2000, "origin")
     with L.Lattice(dev_channel(ch)) as lat:
@@ -86,6 +91,28 @@ def test_c1_parity_2000_steps(orc, geom, variant):
         assert np.array_equal(rho, fl["rho"])
         assert np.array_equal(ux, fl["ux"])
         assert np.array_equal(uy, fl["uy"])
+
+
+@pytest.mark.parametrize("variant", ["tight", "panel4", "panel6"])
+def test_deep_sweeps_2000_steps(orc, variant):
+    """512x256, disc B, Re 100, 2000 steps against the oracle, bit for bit:
+    the smallest channel shape of the C1 family on which the four- and
+    six-step marches run deep (250 interior rows), so the headline sweep
+    covers the parity horizon of C1 with its own kernels."""
+    ch = OChannel.geometry(512, 256, "B")
+    _, lat_ref, tmp_ref, prho_ref = orc.run(ch, 2000, "origin")
+    with L.Lattice(dev_channel(ch)) as lat:
+        lat.kernel_timing(True)
+        lat.init_equilibrium()
+        lat.step(2000, variant)
+        lo, to, po = lat.download()
+        recs = lat.kernel_times()
+    assert_same(lo, lat_ref, f"512x256 {variant} lattice")
+    assert_same(to, tmp_ref, f"512x256 {variant} tmpLattice")
+    assert np.array_equal(po[2:256], prho_ref[2:256])
+    depth = {"tight": 1, "panel4": 4, "panel6": 6}[variant]
+    launches = max(r["launches"] for r in recs if r["depth"] == depth)
+    assert launches >= 1999 // depth - 1, recs
 
 
 def test_resume_from_uploaded_state(orc):
@@ -142,18 +169,22 @@ def test_diagnostics_match_oracle(orc):
 
 def test_panel_interior_waves(orc):
     """A lattice with several bands of interior waves: the branch-free
-    interior march (one and two columns of loads in flight, 4, 8 and 16
-    waves per workgroup) against the oracle, bit for bit."""
+    interior march against the oracle, bit for bit, at chunk widths that run
+    two columns of loads in flight (even widths: k_wpanel_in<double, 2, 2>,
+    twelve waves per workgroup) and one (odd widths: k_wpanel_in<double, 1,
+    2>, eight waves per workgroup), from the narrowest chunk up.  The panel
+    sweep takes its workgroup size from the instantiation, not from the
+    tile's second knob."""
     ch = OChannel.geometry(300, 260, "B")
     _, lref, tref, pref = orc.run(ch, 13)
-    for cxk, wpg in ((16, 0), (17, 0), (32, 8), (33, 16), (20, 16), (3, 8), (5, 4)):
+    for cxk in (16, 17, 32, 33, 20, 3, 5):
         with L.Lattice(dev_channel(ch)) as lat:
-            lat.set_tile(cxk, wpg)
+            lat.set_tile(cxk, 0)
             lat.init_equilibrium()
             lat.step(13, "panel")
             lo, to, po = lat.download()
-        assert_same(lo, lref, f"panel chunk {cxk} wpg {wpg}")
-        assert_same(to, tref, f"panel chunk {cxk} wpg {wpg} tmp")
+        assert_same(lo, lref, f"panel chunk {cxk}")
+        assert_same(to, tref, f"panel chunk {cxk} tmp")
         assert np.array_equal(po[2:260], pref[2:260])
 
 
