@@ -257,6 +257,20 @@ size_t lbm_hip_footprint_for(const lbm_problem *prob, int variant);
 int lbm_hip_apply_node_op(int which, int n, double *f, const double *p0,
                           const double *p1, double omega);
 
+/* lbm_hip_apply_node_op without any device allocation: no counterpart in
+ * the reference; for callers that cannot allocate device memory (code under
+ * graph capture elsewhere in the process, tight serving loops, harnesses
+ * that forbid hipMalloc).  The n vectors and their parameters are staged
+ * through a fixed device array of LBM_NODE_OP_MAX x 11 doubles (352 KiB),
+ * which every GPU the library runs on holds until the process ends; n above
+ * LBM_NODE_OP_MAX returns LBM_ERR_ARG.  Same operators, same kernel, same
+ * bits as lbm_hip_apply_node_op.  Synchronous (the null stream).  Calls on
+ * one GPU share its array: the library serialises calls from this process,
+ * and calls that overlap on one GPU from elsewhere would race. */
+#define LBM_NODE_OP_MAX 4096
+int lbm_hip_apply_node_op_noalloc(int which, int n, double *f, const double *p0,
+                                  const double *p1, double omega);
+
 /* Measurement support (no counterpart in the reference): time a plain
  * device-to-device copy kernel of `bytes` (read once, written once) over
  * iters launches; the HBM yardstick the roofline figures are read against. */

@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -19,6 +20,19 @@ thread_local std::string lbm_g_err;
 
 namespace {
 size_t elem_size(int storage) { return storage == LBM_STORE_F64 ? 8 : storage == LBM_STORE_F32 ? 4 : 2; }
+}  // namespace
+
+// Staging of lbm_hip_apply_node_op_noalloc: LBM_NODE_OP_MAX population
+// vectors of 9 doubles, then their p0 and their p1 values.  Each GPU holds
+// its own copy at its own address; host code reaches it only through the
+// address hipGetSymbolAddress returns on that device (the array's name in
+// host code is a stand-in, not a device address).
+static __device__ __align__(256) double scratch[LBM_NODE_OP_MAX * 11];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu[MAX_DEVICES];      // one per device: serialises the calls
+double *g_scratch_ptr[MAX_DEVICES];        // device address of `scratch`, per device
 }  // namespace
 
 struct lbm_ctx {
@@ -787,6 +801,36 @@ int lbm_hip_apply_node_op(int which, int n, double *f, const double *p0, const d
     if (e == hipSuccess) e = hipMemcpy(f, d, (size_t)n * 9 * sizeof(double), hipMemcpyDeviceToHost);
     (void)hipFree(d);
     if (e != hipSuccess) return hip_fail(e, "node operator");
+    return LBM_OK;
+}
+
+int lbm_hip_apply_node_op_noalloc(int which, int n, double *f, const double *p0,
+                                  const double *p1, double omega) {
+    if (which < 0 || which > 17 || n < 0 || (n > 0 && (!f || !p0 || !p1)))
+        return fail(LBM_ERR_ARG, "bad node-operator call");
+    if (n > LBM_NODE_OP_MAX)
+        return fail(LBM_ERR_ARG, "n = " + std::to_string(n) + " exceeds LBM_NODE_OP_MAX (" +
+                                     std::to_string(LBM_NODE_OP_MAX) + ")");
+    if (n == 0) return LBM_OK;
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(LBM_ERR_UNSUPPORTED, "device index too large");
+    std::lock_guard<std::mutex> lock(g_scratch_mu[dev]);
+    if (!g_scratch_ptr[dev]) {
+        // first call on this device, with it current: its copy's address
+        void *ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = static_cast<double *>(ptr);
+    }
+    double *d = g_scratch_ptr[dev];
+    const size_t nn = (size_t)n;
+    hipStream_t st = nullptr;
+    HIPCHK(hipMemcpyAsync(d, f, nn * 9 * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d + 9 * nn, p0, nn * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d + 10 * nn, p1, nn * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(launch_node_op_f64(which, n, d, d + 9 * nn, d + 10 * nn, omega, st));
+    HIPCHK(hipMemcpyAsync(f, d, nn * 9 * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
     return LBM_OK;
 }
 
