@@ -364,7 +364,7 @@ int lbm_hip_batch_create(const lbm_problem *probs, int n, lbm_batch **out) {
         e.p.omega = p.omega;
         e.p.zgb = p.zgb;
         e.off = off;
-        off += (9 * e.p.lay.plane + 255) / 256 * 256;
+        off += (e.p.lay.span() + 255) / 256 * 256;
         param_off[c] = poff;
         poff += 4 * (int64_t)(p.lx + 2) + 4 * (int64_t)(p.ly + 2) + (p.ly + 2);
         b->cells += (int64_t)p.lx * p.ly;
@@ -453,7 +453,7 @@ int lbm_hip_batch_init_equilibrium(lbm_batch *b, const double *const *init_ux) {
             HIPCHK(hipMemsetAsync(b->pool[k], 0, (size_t)b->pool_elems * 8, b->stream));
         } else {
             for (int c = 0; c < b->n; ++c)
-                HIPCHK(launch_fill_zero_f(b->storage, b->ents[c].p.lay.plane,
+                HIPCHK(launch_fill_zero_f(b->storage, b->ents[c].p.lay,
                                           (char *)b->pool[k] + b->ents[c].off * esize(b->storage),
                                           b->stream));
         }
@@ -577,7 +577,7 @@ int lbm_hip_batch_download(lbm_batch *b, int c, double *lattice, double *tmp,
     HIPCHK(hipStreamSynchronize(b->stream));
     double *wide = nullptr;
     if (b->storage != LBM_STORE_F64)
-        HIPCHK(hipMalloc((void **)&wide, 9 * (size_t)L.plane * sizeof(double)));
+        HIPCHK(hipMalloc((void **)&wide, (size_t)L.span() * sizeof(double)));
     int rc = LBM_OK;
     void *srcs[2] = {lat_src, tmp_src};
     double *hosts[2] = {lattice, tmp};
@@ -585,7 +585,7 @@ int lbm_hip_batch_download(lbm_batch *b, int c, double *lattice, double *tmp,
         if (!hosts[k]) continue;
         double *dev = (double *)srcs[k];
         if (wide) {
-            hipError_t ee = launch_widen(b->storage, L.plane, srcs[k], wide, b->stream);
+            hipError_t ee = launch_widen(b->storage, L, srcs[k], wide, b->stream);
             if (ee == hipSuccess) ee = hipStreamSynchronize(b->stream);
             if (ee != hipSuccess) { rc = hip_fail(ee, "widen"); break; }
             dev = wide;

@@ -206,7 +206,7 @@ hipError_t deep_launch(lbm_ctx *c, int from, int levels) {
 // allocate a further population buffer on first use (counted in the footprint)
 int extra_buffer(lbm_ctx *c, void **b) {
     if (*b) return LBM_OK;
-    size_t bytes = 9 * (size_t)c->prob.lay.plane * elem_size(c->storage);
+    size_t bytes = (size_t)c->prob.lay.span() * elem_size(c->storage);
     HIPCHK(hipMalloc(b, bytes));
     c->footprint += bytes;
     return LBM_OK;
@@ -329,6 +329,8 @@ int step_impl(lbm_ctx *c, int variant, int nsteps) {
         // with one fused single step
         if (c->prob.lay.lx < 4)
             return fail(LBM_ERR_UNSUPPORTED, "two-step variants need lx >= 4");
+        if (variant == LBM_PANEL && !wave_grid_fits(c->prob.lay))
+            return fail(LBM_ERR_UNSUPPORTED, "lattice too tall for the panel sweeps");
         int k = 0;
         if (!c->collided) {
             int rc = to_collided(c);
@@ -390,12 +392,12 @@ int transfer(lbm_ctx *c, double *host, int k, bool to_device) {
     if (c->storage == LBM_STORE_F64) return transfer_f64(c->prob.lay, host, dbuf(c, k), to_device);
     const Layout &L = c->prob.lay;
     double *tmp = nullptr;
-    HIPCHK(hipMalloc((void **)&tmp, 9 * (size_t)L.plane * sizeof(double)));
-    hipError_t e = launch_widen(c->storage, L.plane, c->buf[k], tmp, c->stream);
+    HIPCHK(hipMalloc((void **)&tmp, (size_t)L.span() * sizeof(double)));
+    hipError_t e = launch_widen(c->storage, L, c->buf[k], tmp, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     int rc = e == hipSuccess ? transfer_f64(c->prob.lay, host, tmp, to_device) : hip_fail(e, "widen");
     if (!rc && to_device) {
-        e = launch_narrow(c->storage, L.plane, tmp, c->buf[k], c->stream);
+        e = launch_narrow(c->storage, L, tmp, c->buf[k], c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) rc = hip_fail(e, "narrow");
     }
@@ -412,7 +414,7 @@ const char *lbm_hip_last_error(void) { return lbm_g_err.c_str(); }
 size_t lbm_hip_footprint_of(const lbm_problem *p) {
     if (!p || p->lx < 1 || p->ly < 1) return 0;
     Layout L = make_layout(p->lx, p->ly);
-    size_t pops = 2 * 9 * (size_t)L.plane * elem_size(p->storage);
+    size_t pops = 2 * (size_t)L.span() * elem_size(p->storage);
     size_t sides = 2 * sizeof(double) * (2 * (size_t)(p->lx + 2) + 2 * (size_t)(p->ly + 2));
     size_t diag = 4 * sizeof(double) * (size_t)diag_partials_needed(L) + 64;
     return pops + sides + (size_t)(p->ly + 2) * sizeof(double) + diag;
@@ -422,7 +424,7 @@ size_t lbm_hip_footprint_for(const lbm_problem *p, int variant) {
     const size_t base = lbm_hip_footprint_of(p);
     if (!base) return 0;
     const size_t extra = variant == LBM_PANEL6 ? 2 : variant == LBM_PANEL4 ? 1 : 0;
-    return base + extra * 9 * (size_t)make_layout(p->lx, p->ly).plane * elem_size(p->storage);
+    return base + extra * (size_t)make_layout(p->lx, p->ly).span() * elem_size(p->storage);
 }
 
 int lbm_hip_create(const lbm_problem *p, lbm_ctx **out) {
@@ -446,7 +448,7 @@ int lbm_hip_create(const lbm_problem *p, lbm_ctx **out) {
         if (ea != hipSuccess) { delete c; return hip_fail(ea, "hipFuncSetAttribute"); }
         lds_attr_set = true;
     }
-    size_t bytes = 9 * (size_t)c->prob.lay.plane * elem_size(p->storage);
+    size_t bytes = (size_t)c->prob.lay.span() * elem_size(p->storage);
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete c; return hip_fail(e, "hipStreamCreate"); }
     e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
@@ -462,7 +464,7 @@ int lbm_hip_create(const lbm_problem *p, lbm_ctx **out) {
         if (e != hipSuccess) return cleanup(e, "hipMalloc populations");
         e = p->storage == LBM_STORE_F64
                 ? hipMemset(c->buf[k], 0, bytes)
-                : launch_fill_zero_f(p->storage, c->prob.lay.plane, c->buf[k], nullptr);
+                : launch_fill_zero_f(p->storage, c->prob.lay, c->buf[k], nullptr);
         if (e == hipSuccess) e = hipDeviceSynchronize();
         if (e != hipSuccess) return cleanup(e, "clearing populations");
     }
@@ -529,7 +531,7 @@ int lbm_hip_destroy(lbm_ctx *c) {
 int lbm_hip_init_equilibrium(lbm_ctx *c, const double *init_ux) {
     if (!c || !init_ux) return fail(LBM_ERR_ARG, "null argument");
     const Layout &L = c->prob.lay;
-    size_t bytes = 9 * (size_t)L.plane * elem_size(c->storage);
+    size_t bytes = (size_t)L.span() * elem_size(c->storage);
     double *d_ux = nullptr;
     HIPCHK(hipMalloc((void **)&d_ux, (L.ly + 2) * sizeof(double)));
     hipError_t e = hipMemcpy(d_ux, init_ux, (L.ly + 2) * sizeof(double), hipMemcpyHostToDevice);
@@ -537,7 +539,7 @@ int lbm_hip_init_equilibrium(lbm_ctx *c, const double *init_ux) {
     // that is g = -w, not a zero bit pattern
     for (int k = 0; k < 2 && e == hipSuccess; ++k)
         e = c->storage == LBM_STORE_F64 ? hipMemsetAsync(c->buf[k], 0, bytes, c->stream)
-                                        : launch_fill_zero_f(c->storage, L.plane, c->buf[k],
+                                        : launch_fill_zero_f(c->storage, L, c->buf[k],
                                                              c->stream);
     if (e == hipSuccess)
         e = c->storage == LBM_STORE_F64 ? launch_init_f64(L, d_ux, dbuf(c, 0), c->stream)

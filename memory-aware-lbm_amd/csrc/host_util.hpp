@@ -53,28 +53,32 @@ inline int validate_problem(const lbm_problem *p) {
     return LBM_OK;
 }
 
-// host node layout <-> device planes, interior nodes only
+// host node layout <-> device buffer, interior nodes only.  One population
+// at a time: its lx + 2 column segments (seg contiguous elements each, pitch
+// apart on the device) go through a pinned staging buffer as one strided copy.
 inline int transfer_f64(const lbm::Layout &L, double *host, double *devbuf, bool to_device) {
-    size_t plane_bytes = (size_t)L.plane * sizeof(double);
+    const size_t seg = (size_t)L.segment();
+    const size_t cols = (size_t)L.lx + 2;
+    const size_t seg_bytes = seg * sizeof(double), pitch_bytes = (size_t)L.pitch * sizeof(double);
     double *stage = nullptr;
-    HIPCHK(hipHostMalloc((void **)&stage, plane_bytes, hipHostMallocDefault));
-    int ny = L.ly + 2;
+    HIPCHK(hipHostMalloc((void **)&stage, seg_bytes * cols, hipHostMallocDefault));
+    const int ny = L.ly + 2;
+    auto at = [&](int x, int y) { return (size_t)x * seg + y + L.yoff; };
     hipError_t e = hipSuccess;
     for (int i = 0; i < 9 && e == hipSuccess; ++i) {
         double *dev = devbuf + (int64_t)i * L.plane;
+        // keep the device's ghost ring: read the population first on upload too
+        e = hipMemcpy2D(stage, seg_bytes, dev, pitch_bytes, seg_bytes, cols, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) break;
         if (to_device) {
-            e = hipMemcpy(stage, dev, plane_bytes, hipMemcpyDeviceToHost);  // keep ghosts
-            if (e != hipSuccess) break;
             for (int x = 1; x <= L.lx; ++x)
                 for (int y = 1; y <= L.ly; ++y)
-                    stage[L.node(x, y)] = host[((size_t)x * ny + y) * 9 + i];
-            e = hipMemcpy(dev, stage, plane_bytes, hipMemcpyHostToDevice);
+                    stage[at(x, y)] = host[((size_t)x * ny + y) * 9 + i];
+            e = hipMemcpy2D(dev, pitch_bytes, stage, seg_bytes, seg_bytes, cols, hipMemcpyHostToDevice);
         } else {
-            e = hipMemcpy(stage, dev, plane_bytes, hipMemcpyDeviceToHost);
-            if (e != hipSuccess) break;
             for (int x = 1; x <= L.lx; ++x)
                 for (int y = 1; y <= L.ly; ++y)
-                    host[((size_t)x * ny + y) * 9 + i] = stage[L.node(x, y)];
+                    host[((size_t)x * ny + y) * 9 + i] = stage[at(x, y)];
         }
     }
     hipError_t e2 = hipHostFree(stage);
@@ -82,4 +86,3 @@ inline int transfer_f64(const lbm::Layout &L, double *host, double *devbuf, bool
     if (e2 != hipSuccess) return hip_fail(e2, "hipHostFree");
     return LBM_OK;
 }
-

@@ -56,34 +56,30 @@ __global__ void k_init_lp(Layout L, const double *__restrict__ init_ux, S *__res
 }
 
 // f = 0 everywhere (the reference's calloc'd buffers) in deviation storage:
-// g_i = -w_i on every element of the plane, ghost ring included
+// Flat kernels over a whole buffer of `span` elements; the population of
+// element k is (k / plane) % 9 (planes: plane = one population's elements;
+// interleaved columns: plane = one column's segment of one population).
+__device__ __forceinline__ double w_of(int i) { return i == 0 ? 4. / 9. : (i <= 4 ? 1. / 9. : 1. / 36.); }
+
 template <typename S>
-__global__ void k_fill_zero_f(int64_t plane, S *__restrict__ dst) {
+__global__ void k_fill_zero_f(int64_t span, int64_t plane, S *__restrict__ dst) {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= plane) return;
-    int i = blockIdx.y;
-    double w = i == 0 ? 4. / 9. : (i <= 4 ? 1. / 9. : 1. / 36.);
-    dst[(int64_t)i * plane + k] = (S)(-w);
+    if (k >= span) return;
+    dst[k] = (S)(-w_of((int)((k / plane) % 9)));
 }
 
-// widen a low-precision buffer to FP64 populations (f = w + g), whole planes
 template <typename S>
-__global__ void k_widen(int64_t plane, const S *__restrict__ src, double *__restrict__ dst) {
+__global__ void k_widen(int64_t span, int64_t plane, const S *__restrict__ src, double *__restrict__ dst) {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= plane) return;
-    int i = blockIdx.y;
-    double w = i == 0 ? 4. / 9. : (i <= 4 ? 1. / 9. : 1. / 36.);
-    dst[(int64_t)i * plane + k] = w + (double)(float)src[(int64_t)i * plane + k];
+    if (k >= span) return;
+    dst[k] = w_of((int)((k / plane) % 9)) + (double)(float)src[k];
 }
 
-// narrow FP64 populations to stored deviations
 template <typename S>
-__global__ void k_narrow(int64_t plane, const double *__restrict__ src, S *__restrict__ dst) {
+__global__ void k_narrow(int64_t span, int64_t plane, const double *__restrict__ src, S *__restrict__ dst) {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= plane) return;
-    int i = blockIdx.y;
-    double w = i == 0 ? 4. / 9. : (i <= 4 ? 1. / 9. : 1. / 36.);
-    dst[(int64_t)i * plane + k] = (S)(src[(int64_t)i * plane + k] - w);
+    if (k >= span) return;
+    dst[k] = (S)(src[k] - w_of((int)((k / plane) % 9)));
 }
 
 // ---- launchers ----------------------------------------------------------
@@ -154,24 +150,27 @@ hipError_t launch_diag_lp(int storage, const Problem &p, bool pull, const void *
     return hipGetLastError();
 }
 
-hipError_t launch_fill_zero_f(int storage, int64_t plane, void *dst, hipStream_t st) {
-    dim3 g((unsigned)((plane + 255) / 256), 9);
-    if (storage == 1) k_fill_zero_f<float><<<g, 256, 0, st>>>(plane, (float *)dst);
-    else k_fill_zero_f<_Float16><<<g, 256, 0, st>>>(plane, (_Float16 *)dst);
+hipError_t launch_fill_zero_f(int storage, const Layout &L, void *dst, hipStream_t st) {
+    const int64_t span = L.span();
+    const unsigned g = (unsigned)((span + 255) / 256);
+    if (storage == 1) k_fill_zero_f<float><<<g, 256, 0, st>>>(span, L.plane, (float *)dst);
+    else k_fill_zero_f<_Float16><<<g, 256, 0, st>>>(span, L.plane, (_Float16 *)dst);
     return hipGetLastError();
 }
 
-hipError_t launch_widen(int storage, int64_t plane, const void *src, double *dst, hipStream_t st) {
-    dim3 g((unsigned)((plane + 255) / 256), 9);
-    if (storage == 1) k_widen<float><<<g, 256, 0, st>>>(plane, (const float *)src, dst);
-    else k_widen<_Float16><<<g, 256, 0, st>>>(plane, (const _Float16 *)src, dst);
+hipError_t launch_widen(int storage, const Layout &L, const void *src, double *dst, hipStream_t st) {
+    const int64_t span = L.span();
+    const unsigned g = (unsigned)((span + 255) / 256);
+    if (storage == 1) k_widen<float><<<g, 256, 0, st>>>(span, L.plane, (const float *)src, dst);
+    else k_widen<_Float16><<<g, 256, 0, st>>>(span, L.plane, (const _Float16 *)src, dst);
     return hipGetLastError();
 }
 
-hipError_t launch_narrow(int storage, int64_t plane, const double *src, void *dst, hipStream_t st) {
-    dim3 g((unsigned)((plane + 255) / 256), 9);
-    if (storage == 1) k_narrow<float><<<g, 256, 0, st>>>(plane, src, (float *)dst);
-    else k_narrow<_Float16><<<g, 256, 0, st>>>(plane, src, (_Float16 *)dst);
+hipError_t launch_narrow(int storage, const Layout &L, const double *src, void *dst, hipStream_t st) {
+    const int64_t span = L.span();
+    const unsigned g = (unsigned)((span + 255) / 256);
+    if (storage == 1) k_narrow<float><<<g, 256, 0, st>>>(span, L.plane, src, (float *)dst);
+    else k_narrow<_Float16><<<g, 256, 0, st>>>(span, L.plane, src, (_Float16 *)dst);
     return hipGetLastError();
 }
 

@@ -44,10 +44,30 @@
 
 namespace lbm {
 
-__device__ __forceinline__ double lane_up(double v) { return __shfl_up(v, 1, 64); }
-__device__ __forceinline__ double lane_down(double v) { return __shfl_down(v, 1, 64); }
-__device__ __forceinline__ float lane_up(float v) { return __shfl_up(v, 1, 64); }
-__device__ __forceinline__ float lane_down(float v) { return __shfl_down(v, 1, 64); }
+// Lane shifts by one row through DPP wave shifts (a VALU move, a few cycles)
+// instead of ds_bpermute through the LDS crossbar: the shifted values feed the
+// next level's collision directly, so their latency sits on the march's
+// dependency chain (six-step FP64 march +2%, four-step FP32 +2%, same bits).
+// Lane l takes lane l-1 (wave_shr:1, lane_up) or l+1 (wave_shl:1,
+// lane_down); the lane without a source keeps its own value, as
+// __shfl_up / __shfl_down leave it.
+constexpr int DPP_WAVE_SHL1 = 0x130, DPP_WAVE_SHR1 = 0x138;
+template <int CTRL> __device__ __forceinline__ uint32_t dpp32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL> __device__ __forceinline__ double dpp64(double v) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint64_t lo = dpp32<CTRL>((uint32_t)u), hi = dpp32<CTRL>((uint32_t)(u >> 32));
+    return __builtin_bit_cast(double, (hi << 32) | lo);
+}
+__device__ __forceinline__ double lane_up(double v) { return dpp64<DPP_WAVE_SHR1>(v); }
+__device__ __forceinline__ double lane_down(double v) { return dpp64<DPP_WAVE_SHL1>(v); }
+__device__ __forceinline__ float lane_up(float v) {
+    return __builtin_bit_cast(float, dpp32<DPP_WAVE_SHR1>(__builtin_bit_cast(uint32_t, v)));
+}
+__device__ __forceinline__ float lane_down(float v) {
+    return __builtin_bit_cast(float, dpp32<DPP_WAVE_SHL1>(__builtin_bit_cast(uint32_t, v)));
+}
 
 // ---- perimeter waves ---------------------------------------------------------
 // Waves whose rows or columns (halo included) touch the channel perimeter
@@ -419,9 +439,11 @@ static WaveGrid wave_grid_max(const Layout &L, int cx, int cmax, int64_t waves) 
         cx = (int)(c < 4 ? 4 : c > cmax ? cmax : c) & ~1;
     }
     g.cx = cx;
-    // the interior kernel's buffer resources span one chunk of one plane and
-    // must stay below the drop offset (2 GiB), at up to 8 bytes per element
-    while (g.cx > 3 && (int64_t)g.cx * L.pitch * 8 >= (int64_t)DROP) g.cx /= 2;
+    // the interior kernel's buffer resources span one chunk's columns and its
+    // 32-bit load offsets the chunk plus its halo columns (at most 2 x 5 + 2
+    // more): both stay below the drop offset (2 GiB) at up to 8 bytes per
+    // element (lattices too tall for 3 columns: wave_grid_fits)
+    while (g.cx > 3 && (int64_t)(g.cx + 12) * L.pitch * 8 >= (int64_t)DROP) g.cx /= 2;
     g.nbands = (L.ly + WAVE_ROWS - 1) / WAVE_ROWS;
     g.ncx = (L.lx + g.cx - 1) / g.cx;
     g.ntasks = g.nbands * g.ncx;
@@ -530,8 +552,11 @@ static WaveGrid wave_grid_deep(const Layout &L, int cx, int storage, int levels)
     return make_thin(L, wave_grid_max(L, cx, cmax, 6144), levels - 1, 2 * WAVE_ROWS);
 }
 
+bool wave_grid_fits(const Layout &L) { return (int64_t)(3 + 12) * L.pitch * 8 < (int64_t)DROP; }
+
 bool wpanel_deep_possible(const Layout &L, int cx, int storage, int levels) {
     if (levels != 4 && levels != 6) return false;
+    if (!wave_grid_fits(L)) return false;
     WaveGrid g = wave_grid_deep(L, cx, storage, levels);
     // an interior of at least 124 rows and two chunks
     return g.thin && g.c_hi - g.c_lo >= 2;

@@ -49,9 +49,9 @@ hipError_t launch_macros_lp(int storage, const Layout &L, bool pull, const void 
                             double *ux, double *uy, hipStream_t st);
 hipError_t launch_diag_lp(int storage, const Problem &p, bool pull, const void *src,
                           double *partial, hipStream_t st);
-hipError_t launch_fill_zero_f(int storage, int64_t plane, void *dst, hipStream_t st);
-hipError_t launch_widen(int storage, int64_t plane, const void *src, double *dst, hipStream_t st);
-hipError_t launch_narrow(int storage, int64_t plane, const double *src, void *dst, hipStream_t st);
+hipError_t launch_fill_zero_f(int storage, const Layout &L, void *dst, hipStream_t st);
+hipError_t launch_widen(int storage, const Layout &L, const void *src, double *dst, hipStream_t st);
+hipError_t launch_narrow(int storage, const Layout &L, const double *src, void *dst, hipStream_t st);
 
 // Two-step (temporally blocked) kernels: kernels_2step_f64.hip
 struct BlockGrid {
@@ -95,6 +95,10 @@ __host__ __device__ inline WaveSpan wave_span(const Layout &L, const WaveGrid &g
 }
 
 WaveGrid wave_grid(const Layout &L, int cx);
+// false for lattices so tall that even three-column chunks overflow the
+// interior kernel's 32-bit offsets (ly above about 1.9 M rows): the register
+// marches refuse them
+bool wave_grid_fits(const Layout &L);
 hipError_t launch_wpanel(int storage, const Problem &p, const WaveGrid &g, const void *P, void *Q,
                          hipStream_t st, hipStream_t side, hipEvent_t fork, hipEvent_t join,
                          KernelProbe *probe = nullptr);

@@ -1,13 +1,21 @@
 // lbm_common.hpp -- device data layout and node classification shared by
 // every kernel of the library.
 //
-// Layout in HBM (structure of arrays, one plane per population):
-//   element (i, x, y) at  i*plane + x*pitch + y + yoff
+// Layout in HBM (structure of arrays with the populations of a column side
+// by side):
+//   element (i, x, y) at  i*plane + x*pitch + y + yoff,  pitch = 9*plane
 // with x = 0..lx+1 and y = 0..ly+1 (a one-node ghost ring, as the
 // reference's (lx+2)*(ly+2) buffers, lb.c:89-100).  y is the contiguous
-// dimension, like the reference's lattice[iX][iY].  yoff = ALIGN-1 places
-// y = 1 on an ALIGN-element boundary, and the pitch is a multiple of ALIGN,
-// so every row's interior starts on a 512-byte (FP64) boundary.
+// dimension, like the reference's lattice[iX][iY]; `plane` is one
+// population's segment of a column (ly + 2 + yoff elements rounded up to
+// ALIGN), and the nine segments of column x follow each other.  yoff =
+// ALIGN-1 places y = 1 on an ALIGN-element boundary, so every segment's
+// interior starts on a 512-byte (FP64) boundary.  A sweep that reads or
+// writes the nine populations of a column then touches one contiguous
+// 9 * plane-element region instead of nine regions a whole population apart:
+// on 16384 x 4096 the fused FP64 step runs 3.8% faster, the two-step tiles
+// 15%, the register marches 1-3%, and a plain nine-stream copy of the same
+// shape moves 5.70 TB/s against 5.39 for nine separate planes (MI355X).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -19,9 +27,9 @@ constexpr int ALIGN = 64;  // elements
 
 struct Layout {
     int lx, ly;
-    int pitch;        // elements per x-row
-    int yoff;         // offset of y = 0 inside a row
-    int64_t plane;    // elements per population plane
+    int pitch;        // elements from column x to column x+1 (all nine populations)
+    int yoff;         // offset of y = 0 inside a population's segment
+    int64_t plane;    // elements from population i to i+1 of the same node
 
     __host__ __device__ int64_t at(int i, int x, int y) const {
         return (int64_t)i * plane + (int64_t)x * pitch + y + yoff;
@@ -29,6 +37,13 @@ struct Layout {
     __host__ __device__ int64_t node(int x, int y) const {
         return (int64_t)x * pitch + y + yoff;
     }
+    // elements of one whole buffer (nine populations, ghost ring included)
+    __host__ __device__ int64_t span() const { return (int64_t)(lx + 2) * pitch; }
+    // contiguous elements of one population in one column
+    __host__ __device__ int64_t segment() const { return plane < pitch ? plane : pitch; }
+    // the node's index in the reference's (lx+2) x (ly+2) arrays: a key that
+    // does not depend on the device layout (FP32 stochastic rounding)
+    __host__ __device__ int64_t key(int x, int y) const { return (int64_t)x * (ly + 2) + y; }
 };
 
 inline Layout make_layout(int lx, int ly) {
@@ -36,9 +51,10 @@ inline Layout make_layout(int lx, int ly) {
     L.lx = lx;
     L.ly = ly;
     L.yoff = ALIGN - 1;
-    int need = L.yoff + ly + 2;
-    L.pitch = (need + ALIGN - 1) / ALIGN * ALIGN;
-    L.plane = (int64_t)(lx + 2) * L.pitch;
+    const int need = L.yoff + ly + 2;
+    const int seg = (need + ALIGN - 1) / ALIGN * ALIGN;
+    L.plane = seg;
+    L.pitch = 9 * seg;
     return L;
 }
 

@@ -98,7 +98,10 @@ __device__ __forceinline__ void bgk_dev(float g[9], float omega) {
 // nearest, 2.6e-5 with FP32 collisions rounded stochastically, and 5.2e-8
 // with FP64 collisions (walls included) rounded stochastically.  The random
 // bits are a hash of (node, collision index, population), so a run is
-// deterministic and every kernel rounds a given node update identically.
+// deterministic and every kernel rounds a given node update identically; the
+// node enters as its index in the reference's (lx+2) x (ly+2) arrays
+// (Layout::key), not as a device address, so the bits do not depend on how
+// the device lays the populations out.
 
 __device__ __forceinline__ uint32_t sr_mix(uint32_t a) {
     a ^= a >> 16;
@@ -112,8 +115,8 @@ __device__ __forceinline__ uint32_t sr_mix(uint32_t a) {
 // 64 random bits per node and collision (two mixing rounds of the node index
 // and the collision index); each population takes its own 7 of them
 struct SrSeed { uint32_t lo, hi; };
-__device__ __forceinline__ SrSeed sr_seed(int64_t n, uint32_t step) {
-    const uint32_t a = sr_mix((uint32_t)n * 0x9e3779b1u + step * 0x85ebca77u);
+__device__ __forceinline__ SrSeed sr_seed(int64_t key, uint32_t step) {
+    const uint32_t a = sr_mix((uint32_t)key * 0x9e3779b1u + step * 0x85ebca77u);
     return {a, sr_mix(a ^ 0x632be5abu)};
 }
 

@@ -128,7 +128,7 @@ __device__ __forceinline__ void node_step_lp(const Problem &p, int x, int y,
     for_pops([&]<int I>() {
         g[I] = PULL ? ld(&src[I * L.plane + n - cx(I) * L.pitch - cy(I)]) : ld(&src[I * L.plane + n]);
     });
-    const SrSeed h = sizeof(S) == 4 ? sr_seed(n, step) : SrSeed{0u, 0u};
+    const SrSeed h = sizeof(S) == 4 ? sr_seed(L.key(x, y), step) : SrSeed{0u, 0u};
     int s = side_of(p, x, y);
     if (s >= 0) perimeter_lp<S, PULL, RZ>(p, s, x, y, g, src, dst, h);
     else if (p.r2 >= 0 && in_disc(p, x, y)) bounce_dev(g);

@@ -66,7 +66,7 @@ __device__ __forceinline__ void collide_any(const Problem &p, int s, int x, int 
     if constexpr (sizeof(S) == 8) {
         collide_class(p, s, x, y, h, right_p0);
     } else {
-        const SrSeed hs = sizeof(S) == 4 ? sr_seed(p.lay.node(x, y), step) : SrSeed{0u, 0u};
+        const SrSeed hs = sizeof(S) == 4 ? sr_seed(p.lay.key(x, y), step) : SrSeed{0u, 0u};
         if (s < 0) {
             if (p.r2 >= 0 && in_disc(p, x, y)) bounce_dev(h);
             else bgk_lp<S, true>(h, p.omega, hs);
