@@ -74,6 +74,9 @@ def parse():
     ap.add_argument("--variants", default="origin,combine,tight_block,panel,panel4",
                     help="variants timed besides the headline one (comma list, or '')")
     ap.add_argument("--other-steps", type=int, default=1000)
+    ap.add_argument("--warm-code", type=int, default=1, choices=[0, 1],
+                    help="load every kernel on a small channel first (0: not, so that a "
+                         "profile holds only the full lattice's dispatches)")
     ap.add_argument("--settle", type=int, default=121,
                     help="steps the headline sweep runs on the lattice before it is "
                          "re-initialised for the warm-up and timed steps (0: none)")
@@ -169,6 +172,7 @@ def summary(r, storage):
     out = {"glups": round(r["glups"], 4), "ms_per_step": round(r["ms_per_step"], 5),
            "kernel": r["kernel"], "steps_per_launch": r["depth"],
            "kernel_avg_ms": round(r["kernel_avg_ms"], 5),
+           "nodes_per_launch": int(r["kernel_nodes"]),
            "achieved_gbs": round(r["achieved"], 1),
            "frac": round(r["achieved"] / HBM_PEAK_GBS, 4),
            "update_equivalent_gbs": round(r["achieved"] * r["depth"], 1),
@@ -327,7 +331,8 @@ def main():
         sys.exit(2)
     import lbm_amd as lbm
 
-    warm_code(lbm)
+    if args.warm_code:
+        warm_code(lbm)
     ch = lbm.Channel.geometry(args.lx, args.ly, "B")
     head = time_variant(lbm, ch, args.variant, args.steps, args.warmup, settle=args.settle)
     others = {}

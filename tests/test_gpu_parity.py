@@ -242,10 +242,11 @@ def test_panel6_against_oracle(orc, geom):
     """Six steps per launch (five levels in registers, the perimeter frame
     through two further buffers) against the oracle, bit for bit, for step
     counts that leave every remainder (a four-step, a two-step launch, a
-    single fused step) and for chunk widths with one and two columns in
-    flight, up to the 64 columns the largest lattices use."""
+    single fused step; 9 and 10 steps run two four-step launches in place of
+    a six- and a two-step one) and for chunk widths with one and two columns
+    in flight, up to the 64 columns the largest lattices use."""
     ch = OChannel.geometry(600, 400, geom)
-    for steps in (6, 7, 8, 9, 13, 19):
+    for steps in (6, 7, 8, 9, 10, 13, 19):
         _, lref, tref, pref = orc.run(ch, steps)
         for cxk in (0, 5, 8, 16, 64):
             with L.Lattice(dev_channel(ch)) as lat:
@@ -257,6 +258,24 @@ def test_panel6_against_oracle(orc, geom):
             assert_same(lo, lref, what)
             assert_same(to, tref, what + " tmp")
             assert np.array_equal(po[2:400], pref[2:400]), what
+
+
+def test_panel6_launch_plan():
+    """Twenty steps of the six-step sweep from a collided state run as
+    6 + 6 + 4 + 4 (a two-step launch costs twice a four-step one per step),
+    and 21 as 6 + 6 + 4 + 4 + 1."""
+    ch = L.Channel(600, 400, 120, 200, 40)
+    for n, want in ((20, {6: 2, 4: 2}), (21, {6: 2, 4: 2, 1: 1}), (18, {6: 3}), (22, {6: 3, 4: 1})):
+        with L.Lattice(ch) as lat:
+            lat.init_equilibrium()
+            lat.step(1, "panel6")
+            lat.kernel_timing(True)
+            lat.step(n, "panel6")
+            recs = lat.kernel_times()
+        got = {}
+        for r in recs:
+            got[r["depth"]] = got.get(r["depth"], 0) + r["launches"]
+        assert got == want, (n, recs)
 
 
 @pytest.mark.parametrize("variant,lx,ly", [
