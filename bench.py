@@ -93,6 +93,11 @@ def parse():
                     help="requests of the serving workload (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrivals per second")
     ap.add_argument("--serve-seed", type=int, default=0)
+    ap.add_argument("--serve-pool", type=int, default=1, choices=[0, 1],
+                    help="contexts from a stream-ordered pool (0: hipMalloc / hipFree)")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="run only the served window (no error re-runs) and print its "
+                         "summary: for kernel traces of the serving loop")
     ap.add_argument("--cpu-baseline", default="auto", choices=["auto", "off"])
     ap.add_argument("--cpu-steps", type=int, default=100,
                     help="timed steps of each reference OpenMP sweep")
@@ -223,9 +228,12 @@ def serving(args):
     import lbm_amd as lbm
     from lbm_amd.serve import SolveServer, poisson_requests, summarize
     reqs = poisson_requests(args.serve, args.serve_rate, seed=args.serve_seed)
-    srv = SolveServer(max_active=8, slice_ms=20.0, graph_steps=50, keep_fields=True)
+    srv = SolveServer(max_active=8, slice_ms=20.0, graph_steps=50, keep_fields=True,
+                      pool_bytes=(8 << 30) if args.serve_pool else None)
     res = srv.run(reqs)
     out = summarize(res)
+    if args.serve_only:
+        return out
     out = {k: round(v, 5) if isinstance(v, float) else v for k, v in out.items()}
     out.update({"rate_per_s": args.serve_rate, "seed": args.serve_seed,
                 "max_active": srv.max_active, "hbm_budget_gb": round(srv.hbm_budget / 1e9, 1),
@@ -248,6 +256,19 @@ def serving(args):
     if errs:
         out["f32_velocity_rel_l2_vs_f64"] = {"median": float(np.median(errs)),
                                              "max": float(np.max(errs))}
+        # the worst FP32 solve again, FP32 and FP64 side by side: its velocity
+        # error against the step count
+        from lbm_amd.precision import lockstep
+        f32 = [req for req in reqs if req.storage == "f32"]
+        worst = f32[int(np.argmax(errs))]
+        sample = max(1, worst.steps // 100)
+        marks = [sample * 10 * k for k in range(1, 10)]
+        r = lockstep(worst.channel, worst.steps, sample, ("f64", "f32"), checkpoints=marks)
+        c = worst.channel
+        out["worst_f32_solve"] = {
+            "rid": worst.rid, "lattice": [c.lx, c.ly], "disc": [c.obst_x, c.obst_y, c.obst_r],
+            "reynolds": round(c.reynolds, 3), "steps": worst.steps,
+            "u_l2_rel_by_steps": {str(e["steps"]): e["u_l2_rel"] for e in r["errors"]["f32"]}}
     out["f64_velocity_error"] = "0 (bit-identical FP64 path)"
     return out
 
@@ -331,6 +352,9 @@ def main():
         sys.exit(2)
     import lbm_amd as lbm
 
+    if args.serve_only:
+        print(json.dumps({"serving": serving(args)}), flush=True)
+        return
     if args.warm_code:
         warm_code(lbm)
     ch = lbm.Channel.geometry(args.lx, args.ly, "B")

@@ -112,10 +112,11 @@ typedef struct {
 } lbm_problem;
 
 /* Create a device context for a problem on the current HIP device; allocates
- * the two population buffers (2 * 9 * (lx+2) * pitch elements) and the
- * per-side parameter arrays.  Replaces constructSim + setDynamics
+ * the two population buffers (2 x 9 x (lx+2) x (ly+2) populations, padded)
+ * and the per-side parameter arrays, from the device's pool when
+ * lbm_hip_pool_reserve made one.  Replaces constructSim + setDynamics
  * (lb.c:85-121) for the device.  The populations start at zero, as calloc
- * leaves the reference's buffers. */
+ * leaves the reference's buffers.  Waits only for its own stream. */
 int lbm_hip_create(const lbm_problem *prob, lbm_ctx **out);
 int lbm_hip_destroy(lbm_ctx *ctx);                               /* destructSim, lb.c:109-115 */
 
@@ -178,6 +179,19 @@ int lbm_hip_query(lbm_ctx *ctx);
 
 /* Free and total device memory of the current device (hipMemGetInfo). */
 int lbm_hip_device_memory(size_t *free_bytes, size_t *total_bytes);
+
+/* A stream-ordered memory pool for the contexts of the current device (no
+ * counterpart in the reference; serving support).  After
+ * lbm_hip_pool_reserve, contexts created on this device allocate every
+ * device buffer from the pool on their own stream and return it there when
+ * destroyed, so creating and destroying contexts never waits for other
+ * contexts' work (hipMalloc / hipFree do); memory a destroyed context
+ * returns stays in the pool for the next one.  bytes > 0 grows the pool by
+ * that much up front (0: grow on demand).  Calling it again only grows it.
+ * lbm_hip_pool_release destroys the pool once no context uses it
+ * (LBM_ERR_STATE while some do); later contexts use hipMalloc again. */
+int lbm_hip_pool_reserve(size_t bytes);
+int lbm_hip_pool_release(void);
 
 /* Replace the per-position parameters of one side (e.g. a new inlet profile
  * or, with zgb off, prescribed outlet densities). */

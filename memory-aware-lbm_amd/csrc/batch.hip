@@ -610,10 +610,10 @@ int lbm_hip_batch_macros(lbm_batch *b, int c, double *rho, double *ux, double *u
     HIPCHK(hipMalloc((void **)&d, 3 * n * sizeof(double)));
     void *src = (char *)b->pool[b->cur] + e.off * esize(b->storage);
     hipError_t er = b->storage == LBM_STORE_F64
-                        ? launch_macros_f64(L, b->collided, (const double *)src, d, d + n,
-                                            d + 2 * n, b->stream)
-                        : launch_macros_lp(b->storage, L, b->collided, src, d, d + n, d + 2 * n,
-                                           b->stream);
+                        ? launch_macros_f64(L, 1, L.lx, b->collided, (const double *)src, d,
+                                            d + n, d + 2 * n, b->stream)
+                        : launch_macros_lp(b->storage, L, 1, L.lx, b->collided, src, d, d + n,
+                                           d + 2 * n, b->stream);
     if (er == hipSuccess) er = hipStreamSynchronize(b->stream);
     if (er == hipSuccess && rho) er = hipMemcpy(rho, d, n * sizeof(double), hipMemcpyDeviceToHost);
     if (er == hipSuccess && ux) er = hipMemcpy(ux, d + n, n * sizeof(double), hipMemcpyDeviceToHost);

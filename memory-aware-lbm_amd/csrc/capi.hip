@@ -20,6 +20,17 @@ thread_local std::string lbm_g_err;
 
 namespace {
 size_t elem_size(int storage) { return storage == LBM_STORE_F64 ? 8 : storage == LBM_STORE_F32 ? 4 : 2; }
+constexpr int MAX_DEVICES = 64;
+// Stream-ordered memory pools of the library, per device (lbm_hip_pool_reserve)
+std::mutex g_pool_mu;
+hipMemPool_t g_pool[MAX_DEVICES];
+int g_pool_users[MAX_DEVICES];
+// the moments scratch of a context holds at most this many nodes' fields
+constexpr int64_t FIELDS_CHUNK_NODES = 4 << 20;
+int64_t fields_nodes_of(int lx, int ly) {
+    const int64_t n = (int64_t)lx * ly;
+    return n < FIELDS_CHUNK_NODES ? n : (FIELDS_CHUNK_NODES / ly > 0 ? FIELDS_CHUNK_NODES / ly : 1) * (int64_t)ly;
+}
 }  // namespace
 
 // Staging of lbm_hip_apply_node_op_noalloc: LBM_NODE_OP_MAX population
@@ -30,7 +41,6 @@ size_t elem_size(int storage) { return storage == LBM_STORE_F64 ? 8 : storage ==
 static __device__ __align__(256) double scratch[LBM_NODE_OP_MAX * 11];
 
 namespace {
-constexpr int MAX_DEVICES = 64;
 std::mutex g_scratch_mu[MAX_DEVICES];      // one per device: serialises the calls
 double *g_scratch_ptr[MAX_DEVICES];        // device address of `scratch`, per device
 }  // namespace
@@ -64,6 +74,10 @@ struct lbm_ctx {
     struct Timed { int depth; std::string name; double nodes; };
     std::vector<Timed> tlog;
     KernelProbe probe;
+    hipMemPool_t pool;             // the device's library pool, or null (hipMalloc)
+    double *d_init;                // ly + 2 initial / inlet velocities
+    double *d_fields;              // moments scratch: 3 x fields_nodes doubles
+    int64_t fields_nodes;
 };
 
 namespace {
@@ -78,14 +92,34 @@ int upload_side(lbm_ctx *c, int s, const lbm_problem *p, const double *p0, const
     if (p0) std::memcpy(a.data(), p0, n * sizeof(double));
     else if (p->side[s].kind == LBM_BC_PRESSURE) std::fill(a.begin(), a.end(), 1.0);
     if (p1) std::memcpy(b.data(), p1, n * sizeof(double));
-    HIPCHK(hipMemcpy(c->d_p0[s], a.data(), n * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->d_p1[s], b.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    // copies on the context's stream, waited for before the host arrays go
+    HIPCHK(hipMemcpyAsync(c->d_p0[s], a.data(), n * sizeof(double), hipMemcpyHostToDevice,
+                          c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_p1[s], b.data(), n * sizeof(double), hipMemcpyHostToDevice,
+                          c->stream));
     if (s == LBM_SIDE_RIGHT)
-        HIPCHK(hipMemcpy(c->d_outlet, a.data(), n * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpyAsync(c->d_outlet, a.data(), n * sizeof(double), hipMemcpyHostToDevice,
+                              c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return LBM_OK;
 }
 
 double *dbuf(lbm_ctx *c, int k) { return static_cast<double *>(c->buf[k]); }
+
+// Device allocations of a context: stream-ordered on the context's stream
+// from the device's library pool when lbm_hip_pool_reserve made one (no
+// device-wide synchronisation when contexts come and go), else hipMalloc /
+// hipFree.
+hipError_t ctx_malloc(lbm_ctx *c, void **p, size_t bytes) {
+    if (c->pool) return hipMallocFromPoolAsync(p, bytes, c->pool, c->stream);
+    return hipMalloc(p, bytes);
+}
+
+void ctx_free(lbm_ctx *c, void *p) {
+    if (!p) return;
+    if (c->pool) (void)hipFreeAsync(p, c->stream);
+    else (void)hipFree(p);
+}
 
 const char *storage_name(int storage) {
     return storage == LBM_STORE_F64 ? "double" : storage == LBM_STORE_F32 ? "float" : "_Float16";
@@ -207,7 +241,7 @@ hipError_t deep_launch(lbm_ctx *c, int from, int levels) {
 int extra_buffer(lbm_ctx *c, void **b) {
     if (*b) return LBM_OK;
     size_t bytes = (size_t)c->prob.lay.span() * elem_size(c->storage);
-    HIPCHK(hipMalloc(b, bytes));
+    HIPCHK(ctx_malloc(c, b, bytes));
     c->footprint += bytes;
     return LBM_OK;
 }
@@ -425,7 +459,8 @@ size_t lbm_hip_footprint_of(const lbm_problem *p) {
     size_t pops = 2 * (size_t)L.span() * elem_size(p->storage);
     size_t sides = 2 * sizeof(double) * (2 * (size_t)(p->lx + 2) + 2 * (size_t)(p->ly + 2));
     size_t diag = 4 * sizeof(double) * (size_t)diag_partials_needed(L) + 64;
-    return pops + sides + (size_t)(p->ly + 2) * sizeof(double) + diag;
+    size_t fields = 3 * sizeof(double) * (size_t)fields_nodes_of(p->lx, p->ly);
+    return pops + sides + 2 * (size_t)(p->ly + 2) * sizeof(double) + diag + fields;
 }
 
 size_t lbm_hip_footprint_for(const lbm_problem *p, int variant) {
@@ -459,6 +494,16 @@ int lbm_hip_create(const lbm_problem *p, lbm_ctx **out) {
     size_t bytes = (size_t)c->prob.lay.span() * elem_size(p->storage);
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete c; return hip_fail(e, "hipStreamCreate"); }
+    {
+        int dev = 0;
+        if (hipGetDevice(&dev) == hipSuccess && dev >= 0 && dev < MAX_DEVICES) {
+            std::lock_guard<std::mutex> lock(g_pool_mu);
+            if (g_pool[dev]) {
+                c->pool = g_pool[dev];
+                ++g_pool_users[dev];
+            }
+        }
+    }
     e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->join, hipEventDisableTiming);
@@ -467,31 +512,33 @@ int lbm_hip_create(const lbm_problem *p, lbm_ctx **out) {
         lbm_hip_destroy(c);
         return hip_fail(err, what);
     };
+    // everything below is ordered on the context's own stream: nothing here
+    // waits for other contexts' work
     for (int k = 0; k < 2; ++k) {
-        e = hipMalloc(&c->buf[k], bytes);
-        if (e != hipSuccess) return cleanup(e, "hipMalloc populations");
+        e = ctx_malloc(c, &c->buf[k], bytes);
+        if (e != hipSuccess) return cleanup(e, "allocating populations");
         e = p->storage == LBM_STORE_F64
-                ? hipMemset(c->buf[k], 0, bytes)
-                : launch_fill_zero_f(p->storage, c->prob.lay, c->buf[k], nullptr);
-        if (e == hipSuccess) e = hipDeviceSynchronize();
+                ? hipMemsetAsync(c->buf[k], 0, bytes, c->stream)
+                : launch_fill_zero_f(p->storage, c->prob.lay, c->buf[k], c->stream);
         if (e != hipSuccess) return cleanup(e, "clearing populations");
     }
     for (int s = 0; s < 4; ++s) {
         int n = side_len(p, s);
-        e = hipMalloc((void **)&c->d_p0[s], n * sizeof(double));
-        if (e != hipSuccess) return cleanup(e, "hipMalloc side");
-        e = hipMalloc((void **)&c->d_p1[s], n * sizeof(double));
-        if (e != hipSuccess) return cleanup(e, "hipMalloc side");
+        e = ctx_malloc(c, (void **)&c->d_p0[s], n * sizeof(double));
+        if (e != hipSuccess) return cleanup(e, "allocating side parameters");
+        e = ctx_malloc(c, (void **)&c->d_p1[s], n * sizeof(double));
+        if (e != hipSuccess) return cleanup(e, "allocating side parameters");
     }
-    e = hipMalloc((void **)&c->d_outlet, (p->ly + 2) * sizeof(double));
-    if (e != hipSuccess) return cleanup(e, "hipMalloc outlet");
-    e = hipMalloc((void **)&c->d_partial, 4 * sizeof(double) * c->npartial);
-    if (e != hipSuccess) return cleanup(e, "hipMalloc partials");
-    e = hipMalloc((void **)&c->d_out4, 4 * sizeof(double));
-    if (e != hipSuccess) return cleanup(e, "hipMalloc out");
-    e = hipMalloc((void **)&c->d_step, sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMemset(c->d_step, 0, sizeof(uint32_t));
-    if (e == hipSuccess) e = hipDeviceSynchronize();
+    e = ctx_malloc(c, (void **)&c->d_outlet, (p->ly + 2) * sizeof(double));
+    if (e == hipSuccess) e = ctx_malloc(c, (void **)&c->d_init, (p->ly + 2) * sizeof(double));
+    if (e != hipSuccess) return cleanup(e, "allocating outlet");
+    e = ctx_malloc(c, (void **)&c->d_partial, 4 * sizeof(double) * c->npartial);
+    if (e != hipSuccess) return cleanup(e, "allocating partials");
+    e = ctx_malloc(c, (void **)&c->d_out4, 4 * sizeof(double));
+    if (e != hipSuccess) return cleanup(e, "allocating diagnostics");
+    e = ctx_malloc(c, (void **)&c->d_step, sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemsetAsync(c->d_step, 0, sizeof(uint32_t), c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return cleanup(e, "step counter");
     for (int s = 0; s < 4; ++s) {
         rc = upload_side(c, s, p, p->side[s].p0, p->side[s].p1);
@@ -513,20 +560,27 @@ int lbm_hip_create(const lbm_problem *p, lbm_ctx **out) {
 
 int lbm_hip_destroy(lbm_ctx *c) {
     if (!c) return LBM_OK;
+    if (c->side) (void)hipStreamSynchronize(c->side);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     drop_graphs(c);
-    for (int k = 0; k < 2; ++k) if (c->buf[k]) (void)hipFree(c->buf[k]);
+    for (int k = 0; k < 2; ++k) ctx_free(c, c->buf[k]);
     for (int s = 0; s < 4; ++s) {
-        if (c->d_p0[s]) (void)hipFree(c->d_p0[s]);
-        if (c->d_p1[s]) (void)hipFree(c->d_p1[s]);
+        ctx_free(c, c->d_p0[s]);
+        ctx_free(c, c->d_p1[s]);
     }
-    if (c->d_outlet) (void)hipFree(c->d_outlet);
-    if (c->d_partial) (void)hipFree(c->d_partial);
-    if (c->d_out4) (void)hipFree(c->d_out4);
-    if (c->d_step) (void)hipFree(c->d_step);
-    if (c->buf3) (void)hipFree(c->buf3);
-    if (c->buf4) (void)hipFree(c->buf4);
-    if (c->side) (void)hipStreamSynchronize(c->side);
+    for (void *p : {(void *)c->d_outlet, (void *)c->d_init, (void *)c->d_partial,
+                    (void *)c->d_out4, (void *)c->d_step, c->buf3, c->buf4,
+                    (void *)c->d_fields})
+        ctx_free(c, p);
+    if (c->pool) {
+        // the pool's frees are ordered on the stream: let them land first
+        if (c->stream) (void)hipStreamSynchronize(c->stream);
+        int dev = 0;
+        if (hipGetDevice(&dev) == hipSuccess && dev >= 0 && dev < MAX_DEVICES) {
+            std::lock_guard<std::mutex> lock(g_pool_mu);
+            if (g_pool[dev] == c->pool && g_pool_users[dev] > 0) --g_pool_users[dev];
+        }
+    }
     for (hipEvent_t e : c->tev) (void)hipEventDestroy(e);
     if (c->fork) (void)hipEventDestroy(c->fork);
     if (c->join) (void)hipEventDestroy(c->join);
@@ -540,9 +594,9 @@ int lbm_hip_init_equilibrium(lbm_ctx *c, const double *init_ux) {
     if (!c || !init_ux) return fail(LBM_ERR_ARG, "null argument");
     const Layout &L = c->prob.lay;
     size_t bytes = (size_t)L.span() * elem_size(c->storage);
-    double *d_ux = nullptr;
-    HIPCHK(hipMalloc((void **)&d_ux, (L.ly + 2) * sizeof(double)));
-    hipError_t e = hipMemcpy(d_ux, init_ux, (L.ly + 2) * sizeof(double), hipMemcpyHostToDevice);
+    double *d_ux = c->d_init;
+    hipError_t e = hipMemcpyAsync(d_ux, init_ux, (L.ly + 2) * sizeof(double),
+                                  hipMemcpyHostToDevice, c->stream);
     // both buffers start as f = 0 (calloc, lb.c:89-92); in deviation storage
     // that is g = -w, not a zero bit pattern
     for (int k = 0; k < 2 && e == hipSuccess; ++k)
@@ -554,7 +608,6 @@ int lbm_hip_init_equilibrium(lbm_ctx *c, const double *init_ux) {
                                         : launch_init_lp(c->storage, L, d_ux, c->buf[0], c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->d_step, 0, sizeof(uint32_t), c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    (void)hipFree(d_ux);
     if (e != hipSuccess) return hip_fail(e, "init_equilibrium");
     c->cur = 0;
     c->collided = 0;
@@ -646,6 +699,58 @@ int lbm_hip_query(lbm_ctx *c) {
     return hip_fail(e, "hipStreamQuery");
 }
 
+int lbm_hip_pool_reserve(size_t bytes) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(LBM_ERR_UNSUPPORTED, "device index too large");
+    std::lock_guard<std::mutex> lock(g_pool_mu);
+    if (!g_pool[dev]) {
+        hipMemPoolProps props = {};
+        props.allocType = hipMemAllocationTypePinned;
+        props.handleTypes = hipMemHandleTypeNone;
+        props.location.type = hipMemLocationTypeDevice;
+        props.location.id = dev;
+        hipMemPool_t pool = nullptr;
+        HIPCHK(hipMemPoolCreate(&pool, &props));
+        // freed blocks stay in the pool for the next context
+        uint64_t keep = UINT64_MAX;
+        hipError_t e = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        if (e != hipSuccess) {
+            (void)hipMemPoolDestroy(pool);
+            return hip_fail(e, "hipMemPoolSetAttribute");
+        }
+        g_pool[dev] = pool;
+        g_pool_users[dev] = 0;
+    }
+    if (bytes > 0) {
+        // grow the pool once: allocate and free `bytes` on a private stream
+        hipStream_t s = nullptr;
+        HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        void *p = nullptr;
+        hipError_t e = hipMallocFromPoolAsync(&p, bytes, g_pool[dev], s);
+        if (e == hipSuccess) e = hipFreeAsync(p, s);
+        hipError_t e2 = hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+        if (e != hipSuccess) return hip_fail(e, "reserving the pool");
+        if (e2 != hipSuccess) return hip_fail(e2, "reserving the pool");
+    }
+    return LBM_OK;
+}
+
+int lbm_hip_pool_release(void) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(LBM_ERR_UNSUPPORTED, "device index too large");
+    std::lock_guard<std::mutex> lock(g_pool_mu);
+    if (!g_pool[dev]) return LBM_OK;
+    if (g_pool_users[dev] > 0)
+        return fail(LBM_ERR_STATE, std::to_string(g_pool_users[dev]) +
+                                       " contexts still allocate from the pool");
+    HIPCHK(hipMemPoolDestroy(g_pool[dev]));
+    g_pool[dev] = nullptr;
+    return LBM_OK;
+}
+
 int lbm_hip_device_memory(size_t *free_bytes, size_t *total_bytes) {
     size_t f = 0, t = 0;
     HIPCHK(hipMemGetInfo(&f, &t));
@@ -679,21 +784,30 @@ int lbm_hip_set_side_params(lbm_ctx *c, int side, const double *p0, const double
 int lbm_hip_macros(lbm_ctx *c, double *rho, double *ux, double *uy) {
     if (!c) return fail(LBM_ERR_ARG, "null context");
     const Layout &L = c->prob.lay;
-    size_t n = (size_t)L.lx * L.ly;
-    double *d = nullptr;
-    HIPCHK(hipMalloc((void **)&d, 3 * n * sizeof(double)));
-    hipError_t e = c->storage == LBM_STORE_F64
-                       ? launch_macros_f64(L, c->collided, dbuf(c, c->cur), d, d + n, d + 2 * n,
-                                           c->stream)
-                       : launch_macros_lp(c->storage, L, c->collided, c->buf[c->cur], d, d + n,
-                                          d + 2 * n, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e == hipSuccess && rho) e = hipMemcpy(rho, d, n * sizeof(double), hipMemcpyDeviceToHost);
-    if (e == hipSuccess && ux) e = hipMemcpy(ux, d + n, n * sizeof(double), hipMemcpyDeviceToHost);
-    if (e == hipSuccess && uy)
-        e = hipMemcpy(uy, d + 2 * n, n * sizeof(double), hipMemcpyDeviceToHost);
-    (void)hipFree(d);
-    if (e != hipSuccess) return hip_fail(e, "macros");
+    // the context's own scratch, whole columns at a time (allocated on the
+    // first call, counted in the footprint)
+    if (!c->d_fields) {
+        c->fields_nodes = fields_nodes_of(L.lx, L.ly);
+        HIPCHK(ctx_malloc(c, (void **)&c->d_fields, 3 * (size_t)c->fields_nodes * sizeof(double)));
+    }
+    const int chunk = (int)(c->fields_nodes / L.ly);
+    double *d = c->d_fields;
+    const size_t cap = (size_t)c->fields_nodes;
+    for (int x0 = 1; x0 <= L.lx; x0 += chunk) {
+        const int nx = x0 + chunk - 1 <= L.lx ? chunk : L.lx - x0 + 1;
+        const size_t m = (size_t)nx * L.ly, at = (size_t)(x0 - 1) * L.ly;
+        HIPCHK(c->storage == LBM_STORE_F64
+                   ? launch_macros_f64(L, x0, nx, c->collided, dbuf(c, c->cur), d, d + cap,
+                                       d + 2 * cap, c->stream)
+                   : launch_macros_lp(c->storage, L, x0, nx, c->collided, c->buf[c->cur], d,
+                                      d + cap, d + 2 * cap, c->stream));
+        double *out[3] = {rho, ux, uy};
+        for (int k = 0; k < 3; ++k)
+            if (out[k])
+                HIPCHK(hipMemcpyAsync(out[k] + at, d + k * cap, m * sizeof(double),
+                                      hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
     return LBM_OK;
 }
 

@@ -23,18 +23,19 @@ __device__ __forceinline__ void load_post_stream(const Layout &L, const S *__res
     });
 }
 
-// computeMacros (lb.c:134-141) on every interior node, x-major output
+// computeMacros (lb.c:134-141) on every interior node of columns x0 ..
+// x0 + gridDim.y - 1, x-major output from column x0
 template <typename S, bool PULL>
-__global__ void k_macros(Layout L, const S *__restrict__ src, double *rho, double *ux,
+__global__ void k_macros(Layout L, int x0, const S *__restrict__ src, double *rho, double *ux,
                          double *uy) {
     int y = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    int x = blockIdx.y + 1;
+    int x = blockIdx.y + x0;
     if (y > L.ly) return;
     double f[9];
     load_post_stream<S, PULL>(L, src, x, y, f);
     double r, u, v;
     macros(f, r, u, v);
-    int64_t o = (int64_t)(x - 1) * L.ly + (y - 1);
+    int64_t o = (int64_t)(x - x0) * L.ly + (y - 1);
     if (rho) rho[o] = r;
     if (ux) ux[o] = u;
     if (uy) uy[o] = v;

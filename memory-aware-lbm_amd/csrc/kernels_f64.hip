@@ -144,12 +144,13 @@ hipError_t launch_init_f64(const Layout &L, const double *init_ux, double *dst, 
     return hipGetLastError();
 }
 
-hipError_t launch_macros_f64(const Layout &L, bool pull, const double *src, double *rho,
-                             double *ux, double *uy, hipStream_t st) {
+hipError_t launch_macros_f64(const Layout &L, int x0, int nx, bool pull, const double *src,
+                             double *rho, double *ux, double *uy, hipStream_t st) {
     int t;
     dim3 g = node_grid(L, t);
-    if (pull) k_macros<double, true><<<g, t, 0, st>>>(L, src, rho, ux, uy);
-    else k_macros<double, false><<<g, t, 0, st>>>(L, src, rho, ux, uy);
+    g.y = nx;
+    if (pull) k_macros<double, true><<<g, t, 0, st>>>(L, x0, src, rho, ux, uy);
+    else k_macros<double, false><<<g, t, 0, st>>>(L, x0, src, rho, ux, uy);
     return hipGetLastError();
 }
 

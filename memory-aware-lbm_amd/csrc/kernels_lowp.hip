@@ -122,16 +122,19 @@ hipError_t launch_init_lp(int storage, const Layout &L, const double *init_ux, v
     return hipGetLastError();
 }
 
-hipError_t launch_macros_lp(int storage, const Layout &L, bool pull, const void *src, double *rho,
-                            double *ux, double *uy, hipStream_t st) {
+hipError_t launch_macros_lp(int storage, const Layout &L, int x0, int nx, bool pull,
+                            const void *src, double *rho, double *ux, double *uy, hipStream_t st) {
     int t;
     dim3 g = grid_lp(L, t);
+    g.y = nx;
     if (storage == 1) {
-        if (pull) k_macros<float, true><<<g, t, 0, st>>>(L, (const float *)src, rho, ux, uy);
-        else k_macros<float, false><<<g, t, 0, st>>>(L, (const float *)src, rho, ux, uy);
+        const float *s = (const float *)src;
+        if (pull) k_macros<float, true><<<g, t, 0, st>>>(L, x0, s, rho, ux, uy);
+        else k_macros<float, false><<<g, t, 0, st>>>(L, x0, s, rho, ux, uy);
     } else {
-        if (pull) k_macros<_Float16, true><<<g, t, 0, st>>>(L, (const _Float16 *)src, rho, ux, uy);
-        else k_macros<_Float16, false><<<g, t, 0, st>>>(L, (const _Float16 *)src, rho, ux, uy);
+        const _Float16 *s = (const _Float16 *)src;
+        if (pull) k_macros<_Float16, true><<<g, t, 0, st>>>(L, x0, s, rho, ux, uy);
+        else k_macros<_Float16, false><<<g, t, 0, st>>>(L, x0, s, rho, ux, uy);
     }
     return hipGetLastError();
 }

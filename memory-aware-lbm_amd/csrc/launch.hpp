@@ -23,8 +23,9 @@ hipError_t launch_collide_f64(const Problem &p, bool pull, const double *src, do
                               hipStream_t st);
 hipError_t launch_stream_f64(const Layout &L, const double *src, double *dst, hipStream_t st);
 hipError_t launch_init_f64(const Layout &L, const double *init_ux, double *dst, hipStream_t st);
-hipError_t launch_macros_f64(const Layout &L, bool pull, const double *src, double *rho,
-                             double *ux, double *uy, hipStream_t st);
+// moments of columns x0 .. x0 + nx - 1 (x-major output from column x0)
+hipError_t launch_macros_f64(const Layout &L, int x0, int nx, bool pull, const double *src,
+                             double *rho, double *ux, double *uy, hipStream_t st);
 hipError_t launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t st);
 // periodic ghost copy on a streamed state (L from T), any storage format
 hipError_t launch_periodic(int storage, const Layout &L, const void *T, void *Lb,
@@ -45,8 +46,8 @@ hipError_t launch_stream_lp(int storage, const Layout &L, const void *src, void 
                             hipStream_t st);
 hipError_t launch_init_lp(int storage, const Layout &L, const double *init_ux, void *dst,
                           hipStream_t st);
-hipError_t launch_macros_lp(int storage, const Layout &L, bool pull, const void *src, double *rho,
-                            double *ux, double *uy, hipStream_t st);
+hipError_t launch_macros_lp(int storage, const Layout &L, int x0, int nx, bool pull,
+                            const void *src, double *rho, double *ux, double *uy, hipStream_t st);
 hipError_t launch_diag_lp(int storage, const Problem &p, bool pull, const void *src,
                           double *partial, hipStream_t st);
 hipError_t launch_fill_zero_f(int storage, const Layout &L, void *dst, hipStream_t st);

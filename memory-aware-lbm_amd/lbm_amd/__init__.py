@@ -92,6 +92,8 @@ def library():
         "lbm_hip_make_periodic": (C.c_int, [vp]),
         "lbm_hip_query": (C.c_int, [vp]),
         "lbm_hip_device_memory": (C.c_int, [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+        "lbm_hip_pool_reserve": (C.c_int, [C.c_size_t]),
+        "lbm_hip_pool_release": (C.c_int, []),
         "lbm_hip_kernel_timing": (C.c_int, [vp, C.c_int]),
         "lbm_hip_kernel_times": (C.c_int, [vp, C.POINTER(_KernelRecord), C.c_int,
                                            C.POINTER(C.c_int)]),
@@ -301,6 +303,18 @@ def device_memory():
     f, t = C.c_size_t(), C.c_size_t()
     _check(library().lbm_hip_device_memory(C.byref(f), C.byref(t)), "device_memory")
     return int(f.value), int(t.value)
+
+
+def pool_reserve(nbytes: int = 0):
+    """Give the current device a stream-ordered memory pool for contexts
+    (grown by nbytes up front): contexts then come and go without waiting
+    for other contexts' work."""
+    _check(library().lbm_hip_pool_reserve(int(nbytes)), "pool_reserve")
+
+
+def pool_release():
+    """Destroy the device's context pool (all its contexts must be gone)."""
+    _check(library().lbm_hip_pool_release(), "pool_release")
 
 
 def footprint_of(ch: "Channel", storage: str = "f64", variant: Optional[str] = None) -> int:

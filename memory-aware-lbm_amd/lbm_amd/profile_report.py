@@ -148,6 +148,42 @@ def groups(disp: List[Tuple[int, str, float]]) -> Dict[str, List[float]]:
     return out
 
 
+def idle_gaps(trace_dir: str) -> Tuple[float, float, List[Tuple[float, float]]]:
+    """Device-wide idle time of a kernel trace: the gaps between the union of
+    all kernels' [start, end) intervals, on every stream together.  Returns
+    (span ms, busy ms, [(gap start ms from the first kernel, gap ms), ...])."""
+    iv = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"]))
+                for r in _rows(trace_dir, "kernel_trace.csv"))
+    if not iv:
+        return 0.0, 0.0, []
+    t0, gaps, busy = iv[0][0], [], 0
+    cs, ce = iv[0]
+    for s, e in iv[1:]:
+        if s > ce:
+            busy += ce - cs
+            gaps.append(((ce - t0) * 1e-6, (s - ce) * 1e-6))
+            cs, ce = s, e
+        else:
+            ce = max(ce, e)
+    busy += ce - cs
+    return (ce - t0) * 1e-6, busy * 1e-6, gaps
+
+
+def gap_report(trace_dir: str, label: str) -> List[str]:
+    span, busy, gaps = idle_gaps(trace_dir)
+    big = sorted(gaps, key=lambda g: -g[1])
+    out = [f"### {label}", "",
+           f"{span:.1f} ms from the first kernel's start to the last one's end; some kernel "
+           f"running for {busy:.1f} ms ({100 * busy / span if span else 0:.1f}%).  Idle gaps: "
+           f"{sum(1 for g in gaps if g[1] >= 0.1)} of at least 0.1 ms, "
+           f"{sum(1 for g in gaps if g[1] >= 1.0)} of at least 1 ms, "
+           f"{sum(1 for g in gaps if g[1] >= 10.0)} of at least 10 ms; idle in gaps of at "
+           f"least 1 ms: {sum(g[1] for g in gaps if g[1] >= 1.0):.1f} ms.", "",
+           "| longest gaps: at ms | gap ms |", "|---|---|"]
+    out += [f"| {t:.1f} | {g:.3f} |" for t, g in big[:10]]
+    return out + [""]
+
+
 def per_kernel(disp) -> "OrderedDict[str, List[float]]":
     out: "OrderedDict[str, List[float]]" = OrderedDict()
     for _, name, v in disp:
@@ -174,8 +210,10 @@ def stype(name: str) -> str:
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--trace", required=True)
-    ap.add_argument("--fetch", required=True)
-    ap.add_argument("--write", required=True)
+    ap.add_argument("--fetch", default=None)
+    ap.add_argument("--write", default=None)
+    ap.add_argument("--gaps", action="append", default=[], metavar="LABEL=DIR",
+                    help="instead: device-wide idle gaps of kernel traces (repeatable)")
     ap.add_argument("--bench", default=None, help="bench.py's JSON line of the traced run")
     ap.add_argument("--nodes", type=int, default=16384 * 4096)
     ap.add_argument("--title", default="Per-kernel time and HBM traffic, 16384x4096 cylinder "
@@ -183,6 +221,16 @@ def main(argv=None):
     ap.add_argument("--command", action="append", default=[],
                     help="a command that produced the inputs (repeatable)")
     a = ap.parse_args(argv)
+    if a.gaps:
+        out = [f"# {a.title}", ""]
+        if a.command:
+            out += ["Commands (from the repository root on one MI355X):", "", "```",
+                    "cd /tmp && export TMPDIR=/tmp && cd /root/repo"] + a.command + ["```", ""]
+        for spec in a.gaps:
+            label, d = spec.split("=", 1)
+            out += gap_report(d, label)
+        print("\n".join(out))
+        return
 
     bench = None
     if a.bench:

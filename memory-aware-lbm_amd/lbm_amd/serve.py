@@ -19,6 +19,11 @@ server loop:
   * on completion takes the solve's diagnostics (and, if asked, its velocity
     field), frees its memory and records arrival -> completion latency.
 
+Contexts allocate from a stream-ordered device memory pool the server
+reserves at start (lbm_hip_pool_reserve), and every call on a context waits
+only for that context's stream, so admitting or completing a solve never
+stalls the solves already running.
+
 A solve's result does not depend on what else is resident: every solve runs
 in its own device context, on the same kernels, so its numbers are the same
 alone and in the mix (bit for bit; tests/test_gpu_serve.py checks it).
@@ -36,7 +41,7 @@ from typing import Deque, Dict, List, Optional
 
 import numpy as np
 
-from . import Channel, Lattice, device_memory, footprint_of
+from . import Channel, Lattice, device_memory, footprint_of, pool_release, pool_reserve
 
 
 @dataclass
@@ -126,11 +131,14 @@ class SolveServer:
     def __init__(self, hbm_budget: Optional[int] = None, max_active: int = 8,
                  slice_ms: float = 20.0, graph_steps: int = 50,
                  est_glups: Dict[str, float] = None, keep_fields: bool = False,
-                 variant: str = "auto"):
+                 variant: str = "auto", pool_bytes: Optional[int] = 8 << 30):
         if hbm_budget is None:
             free, _ = device_memory()
             hbm_budget = int(0.9 * free)
         self.hbm_budget = hbm_budget
+        # device memory the context pool takes up front (None: no pool, every
+        # context uses hipMalloc / hipFree)
+        self.pool_bytes = None if pool_bytes is None else min(int(pool_bytes), hbm_budget)
         self.max_active = max_active
         self.slice_ms = slice_ms
         self.graph_steps = max(1, graph_steps)
@@ -153,6 +161,15 @@ class SolveServer:
         return n
 
     def run(self, requests: List[Request], clock=time.perf_counter) -> List[Result]:
+        if self.pool_bytes is not None:
+            pool_reserve(self.pool_bytes)
+        try:
+            return self._run(requests, clock)
+        finally:
+            if self.pool_bytes is not None:
+                pool_release()
+
+    def _run(self, requests: List[Request], clock) -> List[Result]:
         pending: Deque[Request] = deque(sorted(requests, key=lambda r: r.arrival))
         waiting: List[Request] = []
         active: List[_Active] = []

@@ -76,3 +76,95 @@ def test_late_arrival_overlaps_running_solve():
     # the small solve is admitted while the big one runs and finishes first
     assert res[1].admitted < res[0].finished
     assert res[1].finished < res[0].finished
+
+
+def test_pool_contexts_equal_plain_contexts():
+    """Contexts allocated from the stream-ordered pool give the same bits as
+    hipMalloc'd ones, a destroyed context's memory is reused, and the pool
+    refuses to go while a context still uses it."""
+    ch = L.Channel(300, 75, 60, 37, 6, reynolds=120.0)
+    want = {}
+    for st in ("f64", "f32"):
+        with L.Lattice(ch, storage=st) as lat:
+            lat.init_equilibrium()
+            lat.step(37, "panel")
+            want[st] = lat.macros()
+    L.pool_reserve(64 << 20)
+    try:
+        for rep in range(2):
+            for st in ("f64", "f32"):
+                with L.Lattice(ch, storage=st) as lat:
+                    lat.init_equilibrium()
+                    lat.step(37, "panel")
+                    got = lat.macros()
+                    if rep == 1 and st == "f64":
+                        with pytest.raises(L.LbmError):
+                            L.pool_release()
+                for a, b in zip(got, want[st]):
+                    assert np.array_equal(a, b), (rep, st)
+    finally:
+        L.pool_release()
+    L.pool_release()    # no pool left: a no-op
+
+
+def test_macros_in_column_chunks():
+    """A lattice larger than the moments scratch (4 M nodes) is read back in
+    whole-column chunks (4200 x 1100: 3813 columns, then 387): the fields
+    equal computeMacros (lb.c:134-141, the same grouping and true division)
+    evaluated in numpy on the downloaded post-stream populations, bit for
+    bit, in every column."""
+    ch = L.Channel.geometry(4200, 1100, "B")
+    with L.Lattice(ch) as lat:
+        lat.init_equilibrium()
+        lat.step(3, "combine")
+        rho, ux, uy = lat.macros()
+        f = lat.download()[0][1:-1, 1:-1, :]
+    f0, f1, f2, f3, f4, f5, f6, f7, f8 = (f[..., i] for i in range(9))
+    north = (f2 + f5) + f6
+    mid = (f0 + f1) + f3
+    south = (f4 + f7) + f8
+    r = (north + mid) + south
+    assert np.array_equal(rho, r)
+    assert np.array_equal(ux, (((f1 + f5) + f8) - ((f3 + f6) + f7)) / r)
+    assert np.array_equal(uy, (north - south) / r)
+
+
+def _busy_context():
+    """A context with about a second of queued work (1800 steps of the
+    six-step sweep on 16384 x 4096)."""
+    lat = L.Lattice(L.Channel.geometry(16384, 4096, "B"))
+    lat.init_equilibrium()
+    lat.step(7, "panel6")
+    lat.sync()
+    lat.step(1800, "panel6")
+    return lat
+
+
+def test_contexts_come_and_go_while_others_run():
+    """Admission and completion in the server never wait for the solves
+    already running: with the device's context pool, creating a context,
+    initialising it, stepping it, reading its fields and destroying it all
+    return while another context's queued sweep is still running (that
+    context reports busy afterwards).  lbm_hip_create / init / macros /
+    diagnostics wait for their own stream only."""
+    import time
+    L.pool_reserve(256 << 20)
+    try:
+        big = _busy_context()
+        try:
+            t0 = time.perf_counter()
+            for st in ("f64", "f32"):
+                with L.Lattice(L.Channel(300, 75, 60, 37, 6), storage=st) as lat:
+                    lat.init_equilibrium()
+                    lat.step(20, "combine")
+                    lat.macros()
+                    lat.diagnostics()
+            dt = time.perf_counter() - t0
+            assert not big.done(), "the long sweep had finished: the test proves nothing"
+            big.sync()
+            t_big = time.perf_counter() - t0
+            assert dt < 0.5 * t_big, (dt, t_big)
+        finally:
+            big.close()
+    finally:
+        L.pool_release()
