@@ -4,8 +4,10 @@
  *
  * Each operator's address identifies a node's dynamics when a Simulation is
  * translated for the GPU.  Called directly on one host node, it runs the
- * corresponding device operator on that node (lbm_hip_apply_node_op), so a
- * driver that post-processes on the host gets the device's exact numbers.
+ * corresponding device operator on that node, so a driver that post-processes
+ * on the host gets the device's exact numbers; the node goes through the
+ * library's fixed staging array (lbm_hip_apply_node_op_noalloc: no device
+ * allocation and no device-wide wait per call).
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -13,7 +15,7 @@
 #include "../../include/boundaries.h"
 #include "../../include/lbm_hip.h"
 
-/* boundaries.h order, matching lbm_hip_apply_node_op's `which` */
+/* boundaries.h order, matching lbm_hip_apply_node_op(_noalloc)'s `which` */
 enum {
     OP_BOUNCE = 0,
     OP_ZH_VEL = 1,        /* + side: upper, lower, left, right */
@@ -32,7 +34,7 @@ static double bulk_omega_of(const Dynamics *bulk, const char *fn) {
 }
 
 static void run(int which, double *f, double p0, double p1, double omega, const char *fn) {
-    int rc = lbm_hip_apply_node_op(which, 1, f, &p0, &p1, omega);
+    int rc = lbm_hip_apply_node_op_noalloc(which, 1, f, &p0, &p1, omega);
     if (rc) {
         fprintf(stderr, "%s: device error %d: %s\n", fn, rc, lbm_hip_last_error());
         abort();

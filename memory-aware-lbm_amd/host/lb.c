@@ -142,7 +142,7 @@ double computeEquilibrium(int iPop, double rho, double ux, double uy, double uSq
 /* The per-node operators run the device operator on the one node. */
 void bgk(double *fPop, void *selfData) {               /* lb.c:154-166 */
     double zero = 0.;
-    check(lbm_hip_apply_node_op(17, 1, fPop, &zero, &zero, *(double *)selfData), "bgk");
+    check(lbm_hip_apply_node_op_noalloc(17, 1, fPop, &zero, &zero, *(double *)selfData), "bgk");
 }
 
 /* ------------------------------------------------------------------ */
