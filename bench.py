@@ -187,8 +187,9 @@ def summary(r, storage):
 
 def batching(args):
     """The batching workload: seeded uneven channels, hipGraph replay,
-    autotuned launch shape (two steps per launch over LDS tiles, or one
-    persistent launch per step); FP64, FP32 and FP16 storage."""
+    autotuned launch shape (two steps per launch over LDS tiles or by the
+    register march over every channel's wave grid, or one persistent launch
+    per step); FP64, FP32 and FP16 storage."""
     from lbm_amd.batch import Batch, autotune, channels_from_seed
     chans = channels_from_seed(args.batch, seed=args.batch_seed)
     out = {"channels": len(chans), "seed": args.batch_seed,
@@ -214,7 +215,9 @@ def batching(args):
                             "update_equivalent_gbs": round(glups * bpu, 1),
                             "launch": {"depth": tune["depth"], "tile_cols": tune["tile_cols"],
                                        "ty_cap": tune["ty_cap"], "wgs_per_cu": tune["wgs_per_cu"],
-                                       "graph_steps": tune["graph_steps"]},
+                                       "graph_steps": tune["graph_steps"],
+                                       "sweep": tune.get("sweep", "tight_block"),
+                                       "chunk_cols": tune.get("chunk_cols", 0)},
                             "footprint_gb": round(b.footprint() / 1e9, 3)}
     return out
 

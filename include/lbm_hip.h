@@ -316,6 +316,16 @@ int lbm_hip_batch_set_launch(lbm_batch *b, int wgs_per_cu, int ty_cap, int graph
  * (default 8), [3, 69] for FP32 / FP16 (default 14 / 10); 0 keeps it.  Odd
  * remainders and the first step after init_equilibrium run one step. */
 int lbm_hip_batch_set_depth(lbm_batch *b, int depth, int tile_cols);
+/* The sweep of the two-step launches: LBM_TIGHT_BLOCK (the LDS tiles above,
+ * the default) or LBM_PANEL (every channel's waves march its columns with
+ * the first step in wave64 registers, the panel sweep's own per-wave code
+ * over each channel's wave grid of chunk_cols-column chunks, from flat work
+ * tables: one launch for the perimeter waves of every channel, on a second
+ * stream, beside one for the interior waves).  chunk_cols: even, in [4, 64]
+ * (default 6; 0 keeps it).  Same bits either way.  LBM_ERR_UNSUPPORTED for
+ * a channel the panel sweep refuses (lx < 4, or too tall for its 32-bit
+ * offsets). */
+int lbm_hip_batch_set_sweep(lbm_batch *b, int variant, int chunk_cols);
 int lbm_hip_batch_step(lbm_batch *b, int nsteps);
 int lbm_hip_batch_sync(lbm_batch *b);
 int lbm_hip_batch_time(lbm_batch *b, int nsteps, double *ms);

@@ -150,6 +150,15 @@ struct lbm_batch {
     int *d_tile_chan = nullptr;                     // channel of each two-step tile
     int64_t ntiles = 0;
     std::vector<int> ly_of, lx_of;
+    // two-step sweep: LDS tiles (LBM_TIGHT_BLOCK) or the register march
+    // (LBM_PANEL) with its per-channel wave grids and work tables
+    int sweep = LBM_TIGHT_BLOCK;
+    int panel_cx = 6;                               // the register march's chunk width
+    hipStream_t side = nullptr;                     // its perimeter waves' stream
+    hipEvent_t fork = nullptr, join = nullptr;
+    ChannelWaves *d_cw = nullptr;
+    int2 *d_edge_items = nullptr, *d_in_items = nullptr;
+    int nedge_items = 0, nin_wgs = 0;
 };
 
 namespace {
@@ -208,6 +217,46 @@ int build_items(lbm_batch *b) {
     return LBM_OK;
 }
 
+// the register march's per-channel wave grids and work tables
+int build_waves(lbm_batch *b) {
+    std::vector<ChannelWaves> cw(b->n);
+    std::vector<int2> edge, in;
+    const int wpg = batch_wpanel_wpg(b->storage);
+    for (int c = 0; c < b->n; ++c) {
+        const BatchEntry &e = b->ents[c];
+        if (e.p.lay.lx < 4 || !wave_grid_fits(e.p.lay))
+            return fail(LBM_ERR_UNSUPPORTED, "channel " + std::to_string(c) +
+                                                 " is too small or too tall for the register march");
+        cw[c].p = e.p;
+        cw[c].g = wave_grid(e.p.lay, b->panel_cx);
+        cw[c].off = e.off;
+        for (int k = 0; k < cw[c].g.nedge; ++k) edge.push_back(make_int2(c, k));
+        for (int k = 0; k < (cw[c].g.nin + wpg - 1) / wpg; ++k) in.push_back(make_int2(c, k));
+    }
+    if (edge.size() > INT32_MAX || in.size() > INT32_MAX)
+        return fail(LBM_ERR_ARG, "batch has more than 2^31 waves");
+    if (b->d_cw) (void)hipFree(b->d_cw);
+    if (b->d_edge_items) (void)hipFree(b->d_edge_items);
+    if (b->d_in_items) (void)hipFree(b->d_in_items);
+    b->d_cw = nullptr;
+    b->d_edge_items = b->d_in_items = nullptr;
+    HIPCHK(hipMalloc((void **)&b->d_cw, cw.size() * sizeof(ChannelWaves)));
+    HIPCHK(hipMemcpy(b->d_cw, cw.data(), cw.size() * sizeof(ChannelWaves), hipMemcpyHostToDevice));
+    if (!edge.empty()) {
+        HIPCHK(hipMalloc((void **)&b->d_edge_items, edge.size() * sizeof(int2)));
+        HIPCHK(hipMemcpy(b->d_edge_items, edge.data(), edge.size() * sizeof(int2),
+                         hipMemcpyHostToDevice));
+    }
+    if (!in.empty()) {
+        HIPCHK(hipMalloc((void **)&b->d_in_items, in.size() * sizeof(int2)));
+        HIPCHK(hipMemcpy(b->d_in_items, in.data(), in.size() * sizeof(int2),
+                         hipMemcpyHostToDevice));
+    }
+    b->nedge_items = (int)edge.size();
+    b->nin_wgs = (int)in.size();
+    return LBM_OK;
+}
+
 hipError_t launch_step(lbm_batch *b, bool pull) {
     int nwg = std::max(1, b->num_cu * b->wgs_per_cu);
     const void *src = b->pool[b->cur];
@@ -235,6 +284,13 @@ hipError_t launch_pair(lbm_batch *b) {
     const size_t lds = batch_block_lds(b->tile_cols, b->storage);
     const void *src = b->pool[b->cur];
     void *dst = b->pool[1 - b->cur];
+    if (b->sweep == LBM_PANEL) {
+        hipError_t e = launch_batch_wpanel(b->storage, b->d_cw, b->d_edge_items, b->nedge_items,
+                                           b->d_in_items, b->nin_wgs, src, dst, b->stream,
+                                           b->side, b->fork, b->join);
+        if (e == hipSuccess && b->storage != LBM_STORE_F64) e = launch_bump(b->d_step, 2, b->stream);
+        return e;
+    }
     switch (b->storage) {
     case LBM_STORE_F64:
         k_batch_block<double><<<grid, BLOCK_THREADS, lds, b->stream>>>(
@@ -377,7 +433,10 @@ int lbm_hip_batch_create(const lbm_problem *probs, int n, lbm_batch **out) {
         return hip_fail(e, what);
     };
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
-    if (e != hipSuccess) return bail(e, "hipStreamCreate");
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->join, hipEventDisableTiming);
+    if (e != hipSuccess) return bail(e, "stream/event create");
     for (const void *k : {(const void *)k_batch_block<double>, (const void *)k_batch_block<float>,
                           (const void *)k_batch_block<_Float16>})
         if (e == hipSuccess)
@@ -438,7 +497,14 @@ int lbm_hip_batch_destroy(lbm_batch *b) {
     if (b->d_ents) (void)hipFree(b->d_ents);
     if (b->d_item_chan) (void)hipFree(b->d_item_chan);
     if (b->d_tile_chan) (void)hipFree(b->d_tile_chan);
+    if (b->d_cw) (void)hipFree(b->d_cw);
+    if (b->d_edge_items) (void)hipFree(b->d_edge_items);
+    if (b->d_in_items) (void)hipFree(b->d_in_items);
     if (b->d_step) (void)hipFree(b->d_step);
+    if (b->side) (void)hipStreamSynchronize(b->side);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->join) (void)hipEventDestroy(b->join);
+    if (b->side) (void)hipStreamDestroy(b->side);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return LBM_OK;
@@ -521,6 +587,26 @@ int lbm_hip_batch_set_depth(lbm_batch *b, int depth, int tile_cols) {
         b->tile_cols = tile_cols;
         return build_items(b);
     }
+    return LBM_OK;
+}
+
+int lbm_hip_batch_set_sweep(lbm_batch *b, int variant, int chunk_cols) {
+    if (!b || (variant != LBM_TIGHT_BLOCK && variant != LBM_PANEL))
+        return fail(LBM_ERR_ARG, "a batch's two-step sweep is LBM_TIGHT_BLOCK or LBM_PANEL");
+    if (chunk_cols != 0 && (chunk_cols < 4 || chunk_cols > 64 || chunk_cols % 2))
+        return fail(LBM_ERR_ARG, "chunk_cols must be even, in [4, 64] (0: keep)");
+    const int cx = chunk_cols ? chunk_cols : b->panel_cx;
+    if (variant == b->sweep && cx == b->panel_cx) return LBM_OK;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    drop_graphs(b);
+    if (variant == LBM_PANEL && (!b->d_cw || cx != b->panel_cx)) {
+        const int old = b->panel_cx;
+        b->panel_cx = cx;
+        const int rc = build_waves(b);
+        if (rc) { b->panel_cx = old; return rc; }
+    }
+    b->panel_cx = cx;
+    b->sweep = variant;
     return LBM_OK;
 }
 

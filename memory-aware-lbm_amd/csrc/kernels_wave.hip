@@ -195,12 +195,13 @@ __device__ __forceinline__ void edge_march(const Problem &p, const WaveGrid &g, 
     }
 }
 
-// GL: lanes per chunk of the thin strips (g.gl; 4 for the two-step grid's
-// two-row strips, 8 for the deep grids' strips of up to six rows)
+// Perimeter wave e of grid g.  GL: lanes per chunk of the thin strips
+// (g.gl; 4 for the two-step grid's two-row strips, 8 for the deep grids'
+// strips of up to six rows)
 template <typename S, int GL>
-__global__ __launch_bounds__(64) void k_wpanel_edge(Problem p, WaveGrid g, uint32_t dstep,
-                                                    const S *__restrict__ P, S *__restrict__ Q) {
-    int e = blockIdx.x;
+__device__ __forceinline__ void wpanel_edge_wave(const Problem &p, const WaveGrid &g, int e,
+                                                 uint32_t dstep, const S *__restrict__ P,
+                                                 S *__restrict__ Q) {
     if (g.thin) {
         // waves [0, nthin): the strips below r0 and from r1 up, 64 / GL
         // chunks each; then the edge chunks of rows [r0, r1) in 62-row bands
@@ -223,6 +224,12 @@ __global__ __launch_bounds__(64) void k_wpanel_edge(Problem p, WaveGrid g, uint3
     int band, ck;
     edge_task(g, e, band, ck);
     edge_march<S, 64>(p, g, dstep, P, Q, 1 + band * WAVE_ROWS, WAVE_ROWS, ck);
+}
+
+template <typename S, int GL>
+__global__ __launch_bounds__(64) void k_wpanel_edge(Problem p, WaveGrid g, uint32_t dstep,
+                                                    const S *__restrict__ P, S *__restrict__ Q) {
+    wpanel_edge_wave<S, GL>(p, g, (int)blockIdx.x, dstep, P, Q);
 }
 
 // ---- interior waves -------------------------------------------------------------
@@ -299,17 +306,18 @@ template <typename S, int DEPTH, int LEVELS> constexpr int wpg_of() {
 // third just computed), then stores level LEVELS of column x-H.
 // DEPTH = columns whose loads are in flight (1 or 2); cx % DEPTH == 0.
 // dstep: index of the launch's first collision relative to the counter.
+// Interior wave t of grid g (t >= g.nin: nothing, the whole wave leaves);
+// the WPG waves of a workgroup march in lockstep, so they take interior
+// waves of one grid.
 template <typename S, int DEPTH, int LEVELS>
-__global__ __launch_bounds__((64 * wpg_of<S, DEPTH, LEVELS>())) void k_wpanel_in(
-    Problem p, WaveGrid g, uint32_t dstep, const S *__restrict__ P, S *__restrict__ Q) {
-    constexpr int WPG = wpg_of<S, DEPTH, LEVELS>();
+__device__ __forceinline__ void wpanel_in_wave(const Problem &p, const WaveGrid &g, const int t,
+                                               uint32_t dstep, const S *__restrict__ P,
+                                               S *__restrict__ Q) {
     using T = typename Calc<S>::T;
     constexpr int H = LEVELS - 1;
     constexpr int ROWS = 64 - 2 * H;
     const Layout &L = p.lay;
     const int lane = threadIdx.x & 63;
-    const int t = __builtin_amdgcn_readfirstlane(
-        xcd_block(blockIdx.x, gridDim.x) * WPG + (threadIdx.x >> 6));
     if (t >= g.nin) return;                              // whole waves only
     const int ck = g.c_lo + t / g.nbl;
     const int Y0 = g.r0 + (t % g.nbl) * ROWS;
@@ -424,6 +432,15 @@ __global__ __launch_bounds__((64 * wpg_of<S, DEPTH, LEVELS>())) void k_wpanel_in
     static_for<DEPTH>([&]<int D>() __attribute__((always_inline)) {
         iter.template operator()<TAIL, LEVELS>(x + D, f[(2 * H + D) % DEPTH]);
     });
+}
+
+template <typename S, int DEPTH, int LEVELS>
+__global__ __launch_bounds__((64 * wpg_of<S, DEPTH, LEVELS>())) void k_wpanel_in(
+    Problem p, WaveGrid g, uint32_t dstep, const S *__restrict__ P, S *__restrict__ Q) {
+    constexpr int WPG = wpg_of<S, DEPTH, LEVELS>();
+    const int t = __builtin_amdgcn_readfirstlane(
+        xcd_block(blockIdx.x, gridDim.x) * WPG + (threadIdx.x >> 6));
+    wpanel_in_wave<S, DEPTH, LEVELS>(p, g, t, dstep, P, Q);
 }
 
 // Chunk width, unless given: the widest even width in [4, 16] that still
@@ -606,6 +623,69 @@ hipError_t launch_wpanel(int storage, const Problem &p, const WaveGrid &g, const
         S *dst = (S *)Q;
         if (depth == 2) launch_in<S, 2, 2>(p, g, 0u, src, dst, st, probe);
         else launch_in<S, 1, 2>(p, g, 0u, src, dst, st, probe);
+    };
+    if (storage == 0) go(double{});
+    else if (storage == 1) go(float{});
+    else go(_Float16{});
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventRecord(join, side);
+    if (e == hipSuccess) e = hipStreamWaitEvent(st, join, 0);
+    return e;
+}
+
+// ---- many channels, two steps per launch -----------------------------------------
+// The batched form of the panel sweep: every channel of a batch has its own
+// two-step wave grid (ChannelWaves); one launch of k_batch_wedge runs every
+// channel's perimeter waves, one of k_batch_win every channel's interior
+// waves, from flat work tables.  An interior workgroup's waves belong to one
+// channel (their lockstep barriers need the same march), and the workgroups
+// of a channel are contiguous in the table, so xcd_block keeps a channel's
+// neighbouring bands in one L2.  Each wave runs the single-lattice panel
+// sweep's own per-wave code, so a channel evolves bit for bit as it would
+// alone.
+
+template <typename S>
+__global__ __launch_bounds__(64) void k_batch_wedge(const ChannelWaves *__restrict__ ch,
+                                                    const int2 *__restrict__ items,
+                                                    const S *__restrict__ A, S *__restrict__ B) {
+    const int2 it = items[blockIdx.x];
+    const ChannelWaves c = ch[__builtin_amdgcn_readfirstlane(it.x)];
+    wpanel_edge_wave<S, 4>(c.p, c.g, __builtin_amdgcn_readfirstlane(it.y), 0u, A + c.off,
+                           B + c.off);
+}
+
+template <typename S>
+__global__ __launch_bounds__((64 * wpg_of<S, 2, 2>())) void k_batch_win(
+    const ChannelWaves *__restrict__ ch, const int2 *__restrict__ items, const S *__restrict__ A,
+    S *__restrict__ B) {
+    constexpr int WPG = wpg_of<S, 2, 2>();
+    const int2 it = items[xcd_block(blockIdx.x, gridDim.x)];
+    const ChannelWaves c = ch[__builtin_amdgcn_readfirstlane(it.x)];
+    const int t = __builtin_amdgcn_readfirstlane(it.y * WPG + (int)(threadIdx.x >> 6));
+    wpanel_in_wave<S, 2, 2>(c.p, c.g, t, 0u, A + c.off, B + c.off);
+}
+
+int batch_wpanel_wpg(int storage) {
+    return storage == 0 ? wpg_of<double, 2, 2>() : storage == 1 ? wpg_of<float, 2, 2>()
+                                                                : wpg_of<_Float16, 2, 2>();
+}
+
+// perimeter waves on `side`, interior waves on `st`, joined back into `st`
+// (they write disjoint nodes), as the single-lattice panel launch
+hipError_t launch_batch_wpanel(int storage, const ChannelWaves *ch, const int2 *edge_items,
+                               int nedge, const int2 *in_items, int nin_wgs, const void *A,
+                               void *B, hipStream_t st, hipStream_t side, hipEvent_t fork,
+                               hipEvent_t join) {
+    hipError_t e = hipEventRecord(fork, st);
+    if (e == hipSuccess) e = hipStreamWaitEvent(side, fork, 0);
+    if (e != hipSuccess) return e;
+    auto go = [&](auto tag) {
+        using S = decltype(tag);
+        if (nedge > 0)
+            k_batch_wedge<S><<<nedge, 64, 0, side>>>(ch, edge_items, (const S *)A, (S *)B);
+        if (nin_wgs > 0)
+            k_batch_win<S><<<nin_wgs, 64 * wpg_of<S, 2, 2>(), 0, st>>>(ch, in_items,
+                                                                      (const S *)A, (S *)B);
     };
     if (storage == 0) go(double{});
     else if (storage == 1) go(float{});

@@ -113,4 +113,21 @@ hipError_t launch_wpanel_deep(int storage, const Problem &p, int cx, int levels,
                               void *const *mid, void *Q, hipStream_t st, hipStream_t side,
                               hipEvent_t fork, hipEvent_t join, KernelProbe *probe = nullptr);
 
+// Many channels of a batch (pool offset `off` elements), two steps per
+// launch with the panel sweep's register march over each channel's wave
+// grid: perimeter waves (edge_items: channel, wave) on `side`, interior
+// waves (in_items: channel, first wave / batch_wpanel_wpg of every
+// workgroup, the workgroups of a channel contiguous) on `st`, joined back
+// into `st`.  The channels' P and Q sit at A + off and B + off.
+struct ChannelWaves {
+    Problem p;
+    WaveGrid g;
+    int64_t off;
+};
+int batch_wpanel_wpg(int storage);
+hipError_t launch_batch_wpanel(int storage, const ChannelWaves *ch, const int2 *edge_items,
+                               int nedge, const int2 *in_items, int nin_wgs, const void *A,
+                               void *B, hipStream_t st, hipStream_t side, hipEvent_t fork,
+                               hipEvent_t join);
+
 }  // namespace lbm

@@ -74,6 +74,7 @@ def _sig(lib):
         "lbm_hip_batch_init_equilibrium": (C.c_int, [vp, C.POINTER(_P)]),
         "lbm_hip_batch_set_launch": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
         "lbm_hip_batch_set_depth": (C.c_int, [vp, C.c_int, C.c_int]),
+        "lbm_hip_batch_set_sweep": (C.c_int, [vp, C.c_int, C.c_int]),
         "lbm_hip_batch_step": (C.c_int, [vp, C.c_int]),
         "lbm_hip_batch_sync": (C.c_int, [vp]),
         "lbm_hip_batch_time": (C.c_int, [vp, C.c_int, _P]),
@@ -105,7 +106,8 @@ class Batch:
         self._h = h
         self.launch = {"wgs_per_cu": 8, "ty_cap": 256, "graph_steps": 0,
                        "depth": 2 if min(c.lx for c in self.channels) >= 4 else 1,
-                       "tile_cols": {"f64": 8, "f32": 14, "f16": 10}[storage]}
+                       "tile_cols": {"f64": 8, "f32": 14, "f16": 10}[storage],
+                       "sweep": "tight_block", "chunk_cols": 6}
 
     def close(self):
         if getattr(self, "_h", None):
@@ -154,6 +156,17 @@ class Batch:
         if tile_cols:
             self.launch["tile_cols"] = tile_cols
 
+    def set_sweep(self, variant: str, chunk_cols: int = 0):
+        """The two-step launches' sweep: "tight_block" (LDS tiles) or
+        "panel" (the register march over every channel's wave grid, chunks
+        of chunk_cols columns; 0 keeps the width)."""
+        from . import VARIANT
+        _check(library().lbm_hip_batch_set_sweep(self._h, VARIANT[variant], int(chunk_cols)),
+               "batch_set_sweep")
+        self.launch["sweep"] = variant
+        if chunk_cols:
+            self.launch["chunk_cols"] = chunk_cols
+
     def step(self, n: int):
         _check(library().lbm_hip_batch_step(self._h, int(n)), "batch_step")
 
@@ -195,7 +208,7 @@ def batch_key(batch: Batch) -> str:
 
 def autotune(batch: Batch, steps: int = 20, graph_steps: int = 10,
              ty_caps=(32, 64, 128, 256), wgs=(2, 4, 8, 16), tile_cols=None,
-             cache_dir: Optional[str] = TUNE_DIR, use_cache: bool = True) -> dict:
+             panel_cols=(6, 8), cache_dir: Optional[str] = TUNE_DIR, use_cache: bool = True) -> dict:
     """Pick the launch shape with the highest throughput on this batch.
 
     Every candidate runs `steps` steps (after one warm-up replay) with
@@ -203,8 +216,9 @@ def autotune(batch: Batch, steps: int = 20, graph_steps: int = 10,
     per-step cost.  Candidates: the one-step launch over ty_caps x wgs and
     the two-step launch over tile_cols (default: widths around three and two
     workgroups per CU for the storage format; skipped when a channel is
-    narrower than four columns).  Returns {"depth",
-    "tile_cols", "ty_cap", "wgs_per_cu", "graph_steps", "glups",
+    narrower than four columns), and the two-step register march over
+    panel_cols-column chunks.  Returns {"depth", "tile_cols", "ty_cap",
+    "wgs_per_cu", "graph_steps", "sweep", "chunk_cols", "glups",
     "candidates": [...]}; cached by batch_key."""
     key = batch_key(batch)
     path = os.path.join(cache_dir, key + ".json") if cache_dir else None
@@ -212,6 +226,7 @@ def autotune(batch: Batch, steps: int = 20, graph_steps: int = 10,
         best = json.load(open(path))
         batch.set_launch(best["wgs_per_cu"], best["ty_cap"], best["graph_steps"])
         batch.set_depth(best.get("depth", 1), best.get("tile_cols", 0))
+        batch.set_sweep(best.get("sweep", "tight_block"), best.get("chunk_cols", 0))
         best["cached"] = True
         return best
     cells = batch.cells
@@ -230,17 +245,29 @@ def autotune(batch: Batch, steps: int = 20, graph_steps: int = 10,
     if tile_cols is None:
         tile_cols = (6, 8, 10, 14) if batch.storage == "f64" else (10, 14, 18)
     if min(c.lx for c in batch.channels) >= 4:
+        batch.set_sweep("tight_block")
         for tc in tile_cols:
             batch.set_depth(2, tc)
             timed(depth=2, tile_cols=tc, ty_cap=batch.launch["ty_cap"],
-                  wgs_per_cu=batch.launch["wgs_per_cu"])
+                  wgs_per_cu=batch.launch["wgs_per_cu"], sweep="tight_block")
+        try:
+            for pc in panel_cols:
+                batch.set_sweep("panel", pc)
+                timed(depth=2, tile_cols=batch.launch["tile_cols"], ty_cap=batch.launch["ty_cap"],
+                      wgs_per_cu=batch.launch["wgs_per_cu"], sweep="panel", chunk_cols=pc)
+        except Exception:       # a channel the register march refuses
+            pass
+    for r in results:
+        r.setdefault("sweep", "tight_block")
+        r.setdefault("chunk_cols", 0)
     best = max(results, key=lambda r: r["glups"])
     out = {"depth": best["depth"], "tile_cols": best["tile_cols"], "ty_cap": best["ty_cap"],
-           "wgs_per_cu": best["wgs_per_cu"], "graph_steps": graph_steps,
-           "glups": best["glups"], "key": key,
+           "wgs_per_cu": best["wgs_per_cu"], "graph_steps": graph_steps, "sweep": best["sweep"],
+           "chunk_cols": best["chunk_cols"], "glups": best["glups"], "key": key,
            "tuned_at": time.strftime("%Y-%m-%d %H:%M:%S"), "candidates": results}
     batch.set_launch(out["wgs_per_cu"], out["ty_cap"], graph_steps)
     batch.set_depth(out["depth"], out["tile_cols"])
+    batch.set_sweep(out["sweep"], out["chunk_cols"])
     if path:
         os.makedirs(cache_dir, exist_ok=True)
         with open(path, "w") as fh:

@@ -153,12 +153,16 @@ def test_seeded_workload_and_autotune(tmp_path):
     with Batch(chans, "f32") as b:
         b.init_equilibrium()
         res = autotune(b, steps=4, graph_steps=2, ty_caps=(64, 256), wgs=(4, 8),
-                       tile_cols=(18,), cache_dir=str(tmp_path))
-        assert res["glups"] > 0 and len(res["candidates"]) == 5
-        assert [c["depth"] for c in res["candidates"]] == [1, 1, 1, 1, 2]
+                       tile_cols=(18,), panel_cols=(6, 8), cache_dir=str(tmp_path))
+        assert res["glups"] > 0 and len(res["candidates"]) == 7
+        assert [c["depth"] for c in res["candidates"]] == [1, 1, 1, 1, 2, 2, 2]
+        assert [c["sweep"] for c in res["candidates"]][-3:] == ["tight_block", "panel", "panel"]
+        assert [c["chunk_cols"] for c in res["candidates"]][-2:] == [6, 8]
         again = autotune(b, cache_dir=str(tmp_path))
         assert again.get("cached") and again["ty_cap"] == res["ty_cap"]
         assert again["depth"] == res["depth"] and b.launch["depth"] == res["depth"]
+        assert again["sweep"] == res["sweep"] and b.launch["sweep"] == res["sweep"]
+        assert again["chunk_cols"] == res["chunk_cols"]
 
 
 def test_autotune_fp64_times_both_depths(tmp_path):
@@ -167,8 +171,67 @@ def test_autotune_fp64_times_both_depths(tmp_path):
     with Batch(chans, "f64") as b:
         b.init_equilibrium()
         res = autotune(b, steps=4, graph_steps=3, ty_caps=(64,), wgs=(4,), tile_cols=(8, 14),
-                       cache_dir=str(tmp_path))
-        assert [c["depth"] for c in res["candidates"]] == [1, 2, 2]
+                       panel_cols=(6,), cache_dir=str(tmp_path))
+        assert [c["depth"] for c in res["candidates"]] == [1, 2, 2, 2]
+        assert res["candidates"][-1]["sweep"] == "panel"
         assert res["depth"] in (1, 2) and b.launch["depth"] == res["depth"]
         b.step(5)
         assert b.steps_done() > 0
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32", "f16"])
+@pytest.mark.parametrize("graph_steps", [0, 6])
+def test_batch_register_march_equals_single_contexts(storage, graph_steps):
+    """The two-step register march over every channel's wave grid (interior
+    waves from one flat work table, the frame around them in two one-step
+    passes through the third pool) gives every channel the bits of the
+    channel alone, eagerly and replayed from hipGraphs."""
+    chans = small_channels() + [L.Channel(700, 250, 140, 125, 25, reynolds=90.0)]
+    steps = 31
+    with Batch(chans, storage) as b:
+        b.set_sweep("panel")
+        b.set_launch(0, 0, graph_steps)
+        b.init_equilibrium()
+        b.step(steps)
+        got = [b.download(i) for i in range(len(chans))]
+    for i, ch in enumerate(chans):
+        want = alone(ch, steps, storage)
+        for g, w, name in zip(got[i], want, ("lattice", "tmp", "outlet")):
+            a, b_ = (g[2:ch.ly], w[2:ch.ly]) if name == "outlet" else (g[1:-1, 1:-1], w[1:-1, 1:-1])
+            assert np.array_equal(a, b_), (i, name, storage, np.abs(a - b_).max())
+
+
+@pytest.mark.parametrize("chunk_cols", [4, 16, 64])
+def test_batch_register_march_chunk_widths(chunk_cols):
+    """Every chunk width cuts the channels into other interior boxes and
+    frames (the widest leaves the small channels all frame): same bits."""
+    chans = small_channels() + [L.Channel(700, 250, 140, 125, 25, reynolds=90.0),
+                                L.Channel(1030, 130, 200, 64, 20)]
+    steps = 17
+    with Batch(chans, "f32") as b:
+        b.set_sweep("panel", chunk_cols)
+        b.init_equilibrium()
+        b.step(steps)
+        got = [b.download(i) for i in range(len(chans))]
+    for i, ch in enumerate(chans):
+        want = alone(ch, steps, "f32")
+        for g, w, name in zip(got[i], want, ("lattice", "tmp", "outlet")):
+            a, b_ = (g[2:ch.ly], w[2:ch.ly]) if name == "outlet" else (g[1:-1, 1:-1], w[1:-1, 1:-1])
+            assert np.array_equal(a, b_), (i, name, chunk_cols)
+
+
+def test_batch_sweep_switches_keep_results(orc):
+    """Switching a batch between the LDS tiles and the register march between
+    steps changes nothing: every channel still equals the oracle."""
+    from oracle_lib import Channel as OC
+    chans = small_channels()[:4]
+    with Batch(chans, "f64") as b:
+        b.init_equilibrium()
+        for n, sweep in ((5, "panel"), (6, "tight_block"), (9, "panel"), (4, "tight_block")):
+            b.set_sweep(sweep)
+            b.step(n)
+        got = [b.download(i)[0] for i in range(len(chans))]
+    for ch, lat in zip(chans, got):
+        oc = OC(ch.lx, ch.ly, ch.obst_x, ch.obst_y, ch.obst_r, ch.u_max, ch.reynolds)
+        _, lref, _, _ = orc.run(oc, 24)
+        assert np.array_equal(lat[1:-1, 1:-1], lref[1:-1, 1:-1])

@@ -1,8 +1,9 @@
 """The batching and serving workloads at their own sizes.
 
 Batching: the 512 channels of seed 0 (lx log-uniform in [128, 2048],
-ly = lx/4, 96.9 M cells, 118 of them with tau < 0.505), advanced at the
-default launch shape with 10-step hipGraphs, as the bench runs them.  The
+ly = lx/4, 96.9 M cells, 118 of them with tau < 0.505), advanced with
+10-step hipGraphs and both two-step sweeps (LDS tiles, register march), as
+the bench runs them.  The
 smallest and the largest channel and those with tau < 0.505 equal the same
 channel alone in its own context bit for bit, in every storage format, and
 three channels (smallest, median, largest) equal the oracle (the reference's
@@ -51,11 +52,13 @@ def test_seeded_batch_shape(seeded):
     assert chans[small].lx < 140 and chans[large].lx > 1900
 
 
+@pytest.mark.parametrize("sweep", ["tight_block", "panel"])
 @pytest.mark.parametrize("storage", ["f64", "f32", "f16"])
-def test_seeded_batch_equals_single_contexts(seeded, storage):
+def test_seeded_batch_equals_single_contexts(seeded, storage, sweep):
     chans, small, _, large, tau = seeded
     ids = [small, large] + (tau if storage == "f64" else tau[:12])
     with Batch(chans, storage) as b:
+        b.set_sweep(sweep)
         b.set_launch(0, 0, 10)
         b.init_equilibrium()
         b.step(STEPS)
@@ -69,11 +72,13 @@ def test_seeded_batch_equals_single_contexts(seeded, storage):
             assert np.array_equal(a, b_), (storage, i, ch, name)
 
 
-def test_seeded_batch_matches_oracle(seeded, orc):
+@pytest.mark.parametrize("sweep", ["tight_block", "panel"])
+def test_seeded_batch_matches_oracle(seeded, orc, sweep):
     from oracle_lib import Channel as OC
     chans, small, median, large, _ = seeded
     ids = [small, median, large]
     with Batch(chans, "f64") as b:
+        b.set_sweep(sweep)
         b.set_launch(0, 0, 10)
         b.init_equilibrium()
         b.step(STEPS)
