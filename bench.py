@@ -87,8 +87,9 @@ def parse():
                     help="channels of the batching workload (0 = skip)")
     ap.add_argument("--batch-seed", type=int, default=0)
     ap.add_argument("--batch-steps", type=int, default=100)
-    ap.add_argument("--batch-tune", default="on", choices=["on", "off"],
-                    help="autotune the batch launch shape (off: the library's defaults)")
+    ap.add_argument("--batch-tune", default="on", choices=["on", "off", "panel"],
+                    help="autotune the batch launch shape (off: the library's defaults, "
+                         "panel: the register march at its default chunk width)")
     ap.add_argument("--serve", type=int, default=40,
                     help="requests of the serving workload (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrivals per second")
@@ -202,6 +203,8 @@ def batching(args):
                 tune = autotune(b, steps=10, graph_steps=10, use_cache=False)
             else:
                 b.set_launch(0, 0, 10)
+                if args.batch_tune == "panel":
+                    b.set_sweep("panel")
                 tune = dict(b.launch)
             b.step(10)
             ms = b.time_steps(args.batch_steps)

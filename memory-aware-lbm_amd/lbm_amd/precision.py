@@ -30,6 +30,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import sys
 import time
 from typing import Dict, Iterable, Sequence
 
@@ -143,6 +144,8 @@ def main(argv=None):
     a = ap.parse_args(argv)
     res = study(a.lx, a.ly, [float(r) for r in a.reynolds.split(",")], a.steps, a.sample,
                 log=lambda m: print(m, flush=True))
+    res["command"] = "cd memory-aware-lbm_amd && python3 -m lbm_amd.precision " + " ".join(
+        sys.argv[1:] if argv is None else argv)
     text = json.dumps(res, indent=1)
     if a.out:
         with open(a.out, "w") as fh:

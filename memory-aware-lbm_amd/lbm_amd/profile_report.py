@@ -6,7 +6,9 @@ Inputs are three rocprofv3 runs of the same command (CSV output):
     rocprofv3 --pmc FETCH_SIZE --output-format csv -d <fetch> -o run -- <cmd>
     rocprofv3 --pmc WRITE_SIZE --output-format csv -d <write> -o run -- <cmd>
 
-and, optionally, the JSON line bench.py printed in the traced run.  The report
+and, optionally, the JSON line bench.py printed in the traced run (with
+--batch-cells N: only the batch kernels of a batching trace of N cells and
+whole two-step launch times, from the trace alone).  The report
 gives, per kernel, the dispatch count and the average / minimum duration, the
 algorithmic bytes of one dispatch (bytes per lattice update -- 144 FP64, 72
 FP32, 36 FP16 -- times the nodes the dispatch updates once, i.e. one read and
@@ -184,6 +186,50 @@ def gap_report(trace_dir: str, label: str) -> List[str]:
     return out + [""]
 
 
+def batch_marches(trace_dir: str) -> Dict[str, List[float]]:
+    """Whole two-step launches of the batched register march, per storage
+    type: a perimeter dispatch (k_batch_wedge, second stream) and the next
+    interior dispatch (k_batch_win) run side by side, so one launch lasts
+    from the earlier start to the later end (ms)."""
+    rows = sorted((int(r["Dispatch_Id"]), short(r["Kernel_Name"]), int(r["Start_Timestamp"]),
+                   int(r["End_Timestamp"])) for r in _rows(trace_dir, "kernel_trace.csv"))
+    out: Dict[str, List[float]] = defaultdict(list)
+    edge = None
+    for _, name, t0, t1 in rows:
+        if name.startswith("k_batch_wedge"):
+            edge = (t0, t1)
+        elif name.startswith("k_batch_win"):
+            s, e = (min(t0, edge[0]), max(t1, edge[1])) if edge else (t0, t1)
+            out[stype(name)].append((e - s) * 1e-6)
+            edge = None
+        else:
+            edge = None
+    return out
+
+
+def batch_report(trace_dir: str, cells: int) -> List[str]:
+    kt = per_kernel(dispatches(trace_dir))
+    out = ["## Kernel durations (rocprofv3 --kernel-trace)", "",
+           "| kernel | calls | avg ms | min ms |", "|---|---|---|---|"]
+    for name, v in sorted(kt.items(), key=lambda kv: -sum(kv[1])):
+        out.append(f"| {name} | {len(v)} | {sum(v) / len(v):.4f} | {min(v):.4f} |")
+    out += ["", "## Whole two-step launches", "",
+            f"Two updates of all {cells:,} cells per launch; algorithmic bytes per launch = "
+            "bytes per lattice update (FP64 144, FP32 72, FP16 36) x 2 x cells.  A register "
+            "march launch lasts from the first start to the last end of its perimeter and "
+            "interior dispatches; an LDS-tile launch is one k_batch_block dispatch.", "",
+            "| launch | launches | avg ms | min ms | algorithmic GB/launch | algorithmic GB/s "
+            "| GLUPS |", "|---|---|---|---|---|---|---|"]
+    rows = [(f"march<{t}>", v) for t, v in batch_marches(trace_dir).items()]
+    rows += [(n, v) for n, v in kt.items() if n.startswith("k_batch_block")]
+    for name, v in rows:
+        avg = sum(v) / len(v)
+        gb = BPU[stype(name)] * 2 * cells / 1e9
+        out.append(f"| {name} | {len(v)} | {avg:.4f} | {min(v):.4f} | {gb:.3f} | "
+                   f"{gb / (avg * 1e-3):.1f} | {2 * cells / (avg * 1e-3) / 1e9:.2f} |")
+    return out
+
+
 def per_kernel(disp) -> "OrderedDict[str, List[float]]":
     out: "OrderedDict[str, List[float]]" = OrderedDict()
     for _, name, v in disp:
@@ -214,6 +260,8 @@ def main(argv=None):
     ap.add_argument("--write", default=None)
     ap.add_argument("--gaps", action="append", default=[], metavar="LABEL=DIR",
                     help="instead: device-wide idle gaps of kernel traces (repeatable)")
+    ap.add_argument("--batch-cells", type=int, default=0,
+                    help="instead: the batch kernels of a batching trace of that many cells")
     ap.add_argument("--bench", default=None, help="bench.py's JSON line of the traced run")
     ap.add_argument("--nodes", type=int, default=16384 * 4096)
     ap.add_argument("--title", default="Per-kernel time and HBM traffic, 16384x4096 cylinder "
@@ -229,6 +277,21 @@ def main(argv=None):
         for spec in a.gaps:
             label, d = spec.split("=", 1)
             out += gap_report(d, label)
+        print("\n".join(out))
+        return
+
+    if a.batch_cells:
+        out = [f"# {a.title}", ""]
+        if a.command:
+            out += ["Commands (from the repository root on one MI355X):", "", "```",
+                    "cd /tmp && export TMPDIR=/tmp && cd /root/repo"] + a.command + ["```", ""]
+        out += batch_report(a.trace, a.batch_cells)
+        if a.bench:
+            text = open(a.bench).read().strip().splitlines()
+            bl = json.loads(text[-1]).get("batching", {}) if text else {}
+            out += ["", "The traced run's own bench line (HIP events over the timed graph "
+                    "replays): " + ", ".join(f"{st} {bl[st]['glups']} GLUPS"
+                                             for st in ("f64", "f32", "f16") if st in bl) + "."]
         print("\n".join(out))
         return
 
