@@ -17,11 +17,12 @@
 //    perimeter included, with no branch.
 //
 // Kernels:
-//  k_collide<false>  streamed -> collided: reads L locally, collides, writes
-//                    the other buffer (one step's collision).
-//  k_collide<true>   collided -> collided: pull gather + collide (one fused
-//                    step), the combine/tight hot kernel.
-//  k_stream          collided -> streamed: pull gather only.
+//  k_collide<double, false>  streamed -> collided: reads L locally, collides,
+//                    writes the other buffer (one step's collision).
+//  k_collide<double, true>   collided -> collided: pull gather + collide (one
+//                    fused step), the combine/tight hot kernel.
+//  k_stream<double>  collided -> streamed: pull gather only.
+//  (these three: rows.hpp, 16 bytes of a column per lane and population)
 //  k_init            equilibrium initial condition.
 //  k_macros / k_diag moments and reductions of the post-stream state.
 //  k_periodic        the periodic ghost copy on a streamed state (any storage).
@@ -29,28 +30,9 @@
 #include "collide_f64.hpp"
 #include "fields.hpp"
 #include "node_step.hpp"
+#include "rows.hpp"
 
 namespace lbm {
-
-template <bool PULL>
-__global__ __launch_bounds__(256) void k_collide(Problem p, const double *__restrict__ src,
-                                                 double *__restrict__ dst) {
-    int y = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    int x = blockIdx.y + 1;
-    if (y > p.lay.ly) return;
-    node_step_f64<PULL>(p, x, y, src, dst);
-}
-
-__global__ __launch_bounds__(256) void k_stream(Layout L, const double *__restrict__ src,
-                                                double *__restrict__ dst) {
-    int y = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    int x = blockIdx.y + 1;
-    if (y > L.ly) return;
-    int64_t n = L.node(x, y);
-    for_pops([&]<int I>() {
-        dst[I * L.plane + n] = src[I * L.plane + n - cx(I) * L.pitch - cy(I)];
-    });
-}
 
 // iniEquilibrium(node, 1, init_ux[y], 0) on every interior node
 // (unsteady.c:164-170, lb.c:71-78).
@@ -124,16 +106,18 @@ static dim3 node_grid(const Layout &L, int &threads) {
 hipError_t launch_collide_f64(const Problem &p, bool pull, const double *src, double *dst,
                               hipStream_t st) {
     int t;
-    dim3 g = node_grid(p.lay, t);
-    if (pull) k_collide<true><<<g, t, 0, st>>>(p, src, dst);
-    else k_collide<false><<<g, t, 0, st>>>(p, src, dst);
+    dim3 g = rows_grid<double>(p.lay, t);
+    if (pull) k_collide<double, true><<<g, t, 0, st>>>(p, src, dst);
+    else k_collide<double, false><<<g, t, 0, st>>>(p, src, dst);
     return hipGetLastError();
 }
 
 hipError_t launch_stream_f64(const Layout &L, const double *src, double *dst, hipStream_t st) {
+    Problem q{};
+    q.lay = L;
     int t;
-    dim3 g = node_grid(L, t);
-    k_stream<<<g, t, 0, st>>>(L, src, dst);
+    dim3 g = rows_grid<double>(L, t);
+    k_stream<double><<<g, t, 0, st>>>(q, src, dst);
     return hipGetLastError();
 }
 

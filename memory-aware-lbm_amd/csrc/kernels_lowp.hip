@@ -20,29 +20,9 @@
 #include "fields.hpp"
 #include "lowp_ops.hpp"
 #include "node_step.hpp"
+#include "rows.hpp"
 
 namespace lbm {
-
-template <typename S, bool PULL>
-__global__ __launch_bounds__(256) void k_collide_lp(Problem p, const S *__restrict__ src,
-                                                    S *__restrict__ dst) {
-    int y = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    int x = blockIdx.y + 1;
-    if (y > p.lay.ly) return;
-    const uint32_t step = sizeof(S) == 4 ? *p.step : 0u;
-    if constexpr (sizeof(S) == 4) fp32_round_toward_zero();
-    node_step_lp<S, PULL, true>(p, x, y, src, dst, step);
-}
-
-template <typename S>
-__global__ __launch_bounds__(256) void k_stream_lp(Layout L, const S *__restrict__ src,
-                                                   S *__restrict__ dst) {
-    int y = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    int x = blockIdx.y + 1;
-    if (y > L.ly) return;
-    int64_t n = L.node(x, y);
-    for_pops([&]<int I>() { dst[I * L.plane + n] = src[I * L.plane + n - cx(I) * L.pitch - cy(I)]; });
-}
 
 template <typename S>
 __global__ void k_init_lp(Layout L, const double *__restrict__ init_ux, S *__restrict__ dst) {
@@ -92,9 +72,19 @@ static dim3 grid_lp(const Layout &L, int &threads) {
 template <typename S>
 static hipError_t collide_t(const Problem &p, bool pull, const void *src, void *dst, hipStream_t st) {
     int t;
-    dim3 g = grid_lp(p.lay, t);
-    if (pull) k_collide_lp<S, true><<<g, t, 0, st>>>(p, (const S *)src, (S *)dst);
-    else k_collide_lp<S, false><<<g, t, 0, st>>>(p, (const S *)src, (S *)dst);
+    dim3 g = rows_grid<S>(p.lay, t);
+    if (pull) k_collide<S, true><<<g, t, 0, st>>>(p, (const S *)src, (S *)dst);
+    else k_collide<S, false><<<g, t, 0, st>>>(p, (const S *)src, (S *)dst);
+    return hipGetLastError();
+}
+
+template <typename S>
+static hipError_t stream_t(const Layout &L, const void *src, void *dst, hipStream_t st) {
+    Problem q{};
+    q.lay = L;
+    int t;
+    dim3 g = rows_grid<S>(L, t);
+    k_stream<S><<<g, t, 0, st>>>(q, (const S *)src, (S *)dst);
     return hipGetLastError();
 }
 
@@ -106,11 +96,7 @@ hipError_t launch_collide_lp(int storage, const Problem &p, bool pull, const voi
 
 hipError_t launch_stream_lp(int storage, const Layout &L, const void *src, void *dst,
                             hipStream_t st) {
-    int t;
-    dim3 g = grid_lp(L, t);
-    if (storage == 1) k_stream_lp<float><<<g, t, 0, st>>>(L, (const float *)src, (float *)dst);
-    else k_stream_lp<_Float16><<<g, t, 0, st>>>(L, (const _Float16 *)src, (_Float16 *)dst);
-    return hipGetLastError();
+    return storage == 1 ? stream_t<float>(L, src, dst, st) : stream_t<_Float16>(L, src, dst, st);
 }
 
 hipError_t launch_init_lp(int storage, const Layout &L, const double *init_ux, void *dst,

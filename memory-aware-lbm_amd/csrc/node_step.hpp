@@ -57,6 +57,20 @@ __device__ __forceinline__ void perimeter_node(const Problem &p, int s, int x, i
 }
 
 
+// The collision of node (x, y) on its post-stream populations f (pulled from
+// a collided src, or read in place from a streamed one), with the perimeter
+// bookkeeping (which writes dst's ghost ring and, on a streamed state, reads
+// dst(x, y) before the caller overwrites it).
+template <bool PULL>
+__device__ __forceinline__ void collide_node_f64(const Problem &p, int x, int y, double f[9],
+                                                 const double *__restrict__ src,
+                                                 double *__restrict__ dst) {
+    int s = side_of(p, x, y);
+    if (s >= 0) perimeter_node<PULL>(p, s, x, y, f, src, dst);
+    else if (p.r2 >= 0 && in_disc(p, x, y)) bounce_back(f);
+    else bgk(f, p.omega);
+}
+
 template <bool PULL>
 __device__ __forceinline__ void node_step_f64(const Problem &p, int x, int y,
                                               const double *__restrict__ src,
@@ -67,10 +81,7 @@ __device__ __forceinline__ void node_step_f64(const Problem &p, int x, int y,
     for_pops([&]<int I>() {
         f[I] = PULL ? src[I * L.plane + n - cx(I) * L.pitch - cy(I)] : src[I * L.plane + n];
     });
-    int s = side_of(p, x, y);
-    if (s >= 0) perimeter_node<PULL>(p, s, x, y, f, src, dst);
-    else if (p.r2 >= 0 && in_disc(p, x, y)) bounce_back(f);
-    else bgk(f, p.omega);
+    collide_node_f64<PULL>(p, x, y, f, src, dst);
     for_pops([&]<int I>() { __builtin_nontemporal_store(f[I], &dst[I * L.plane + n]); });
 }
 
@@ -116,6 +127,20 @@ __device__ __forceinline__ void perimeter_lp(const Problem &p, int s, int x, int
     for_pops([&]<int I>() { g[I] = to_dev<S, I, RZ>(f[I], h); });
 }
 
+// The collision of node (x, y) on its post-stream deviations g, reduced
+// storage (perimeter bookkeeping as collide_node_f64); `step` is the index of
+// this collision (it keys FP32's stochastic rounding); RZ as in sr_f32.
+template <typename S, bool PULL, bool RZ>
+__device__ __forceinline__ void collide_node_lp(const Problem &p, int x, int y, float g[9],
+                                                const S *__restrict__ src, S *__restrict__ dst,
+                                                uint32_t step) {
+    const SrSeed h = sizeof(S) == 4 ? sr_seed(p.lay.key(x, y), step) : SrSeed{0u, 0u};
+    int s = side_of(p, x, y);
+    if (s >= 0) perimeter_lp<S, PULL, RZ>(p, s, x, y, g, src, dst, h);
+    else if (p.r2 >= 0 && in_disc(p, x, y)) bounce_dev(g);
+    else bgk_lp<S, RZ>(g, p.omega, h);
+}
+
 // One collision of node (x, y) with reduced storage; `step` is the index of
 // this collision (it keys FP32's stochastic rounding); RZ as in sr_f32.
 template <typename S, bool PULL, bool RZ>
@@ -128,11 +153,7 @@ __device__ __forceinline__ void node_step_lp(const Problem &p, int x, int y,
     for_pops([&]<int I>() {
         g[I] = PULL ? ld(&src[I * L.plane + n - cx(I) * L.pitch - cy(I)]) : ld(&src[I * L.plane + n]);
     });
-    const SrSeed h = sizeof(S) == 4 ? sr_seed(L.key(x, y), step) : SrSeed{0u, 0u};
-    int s = side_of(p, x, y);
-    if (s >= 0) perimeter_lp<S, PULL, RZ>(p, s, x, y, g, src, dst, h);
-    else if (p.r2 >= 0 && in_disc(p, x, y)) bounce_dev(g);
-    else bgk_lp<S, RZ>(g, p.omega, h);
+    collide_node_lp<S, PULL, RZ>(p, x, y, g, src, dst, step);
     for_pops([&]<int I>() { st_nt(narrow<S>(g[I]), &dst[I * L.plane + n]); });
 }
 
