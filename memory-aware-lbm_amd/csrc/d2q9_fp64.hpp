@@ -43,6 +43,34 @@ __device__ __forceinline__ double rho_of(const double f[9]) {
     return north + mid + south;
 }
 
+// a / d and b / d, each the correctly rounded IEEE quotient: the hardware
+// division sequence (scale, reciprocal and two Newton steps on the
+// denominator, then per quotient a product, its remainder, the fused
+// correction and the special-case fixup), with the denominator's part done
+// once for both quotients.  The denominator's own scale step takes d as the
+// numerator too, which leaves a normal d unscaled, the value the sequence of
+// a / d uses whenever a is zero or at least 2^-969 in magnitude; populations
+// of a lattice give numerators that are zero or far larger than that, so
+// both quotients have the bits of a / d and b / d.
+__device__ __forceinline__ void div_same_den(double a, double b, double d, double &qa,
+                                             double &qb) {
+    FP64_EXACT;
+    bool unused;
+    const double ds = __builtin_amdgcn_div_scale(d, d, false, &unused);
+    const double r0 = __builtin_amdgcn_rcp(ds);
+    const double r1 = __builtin_fma(r0, __builtin_fma(-ds, r0, 1.0), r0);
+    const double r2 = __builtin_fma(r1, __builtin_fma(-ds, r1, 1.0), r1);
+    auto quot = [&](double n) {
+        bool sc;
+        const double ns = __builtin_amdgcn_div_scale(n, d, true, &sc);
+        const double m = ns * r2;
+        const double rem = __builtin_fma(-ds, m, ns);
+        return __builtin_amdgcn_div_fixup(__builtin_amdgcn_div_fmas(rem, r2, m, sc), d, n);
+    };
+    qa = quot(a);
+    qb = quot(b);
+}
+
 __device__ __forceinline__ void macros(const double f[9], double &rho, double &ux,
                                        double &uy) {
     FP64_EXACT;
@@ -50,8 +78,8 @@ __device__ __forceinline__ void macros(const double f[9], double &rho, double &u
     double mid = f[0] + f[1] + f[3];
     double south = f[4] + f[7] + f[8];
     rho = north + mid + south;
-    ux = (f[1] + f[5] + f[8] - (f[3] + f[6] + f[7])) / rho;
-    uy = (north - south) / rho;
+    // ux = (f1 + f5 + f8 - (f3 + f6 + f7)) / rho, uy = (north - south) / rho
+    div_same_den(f[1] + f[5] + f[8] - (f[3] + f[6] + f[7]), north - south, rho, ux, uy);
 }
 
 // Equilibrium (lb.c:144-151).  c.u is formed from integer components; the

@@ -285,10 +285,20 @@ template <typename S, int DEPTH, int LEVELS> constexpr int wpg_of() {
 // Interior wave t of grid g (t >= g.nin: nothing, the whole wave leaves);
 // the WPG waves of a workgroup march in lockstep, so they take interior
 // waves of one grid.
-template <typename S, int DEPTH, int LEVELS>
-__device__ __forceinline__ void wpanel_in_wave(const Problem &p, const WaveGrid &g, const int t,
-                                               uint32_t dstep, const S *__restrict__ P,
-                                               S *__restrict__ Q) {
+// DISC: the wave's nodes may reach the disc (per-lane bounce-back branch).
+// Without that branch an iteration is one basic block, and the scheduler
+// interleaves its collisions so freely that the six-step FP64 march needs
+// 256 registers and spills (1.3x slower); a scheduling fence after each
+// collision keeps the branchy form's register use (205 VGPRs, two waves per
+// SIMD) and still drops its branches and the copies at their merges.
+// Fences that let memory or scalar instructions cross measured the same;
+// unrolling the march by six columns (no window copies at all, 13% fewer
+// instructions per column) 1.5% faster for FP64 and 15% slower for FP32.
+
+template <typename S, int DEPTH, int LEVELS, bool DISC>
+__device__ __forceinline__ void wpanel_in_march(const Problem &p, const WaveGrid &g, const int t,
+                                                uint32_t dstep, const S *__restrict__ P,
+                                                S *__restrict__ Q) {
     using T = typename Calc<S>::T;
     constexpr int H = LEVELS - 1;
     constexpr int ROWS = 64 - 2 * H;
@@ -357,7 +367,8 @@ __device__ __forceinline__ void wpanel_in_wave(const Problem &p, const WaveGrid 
         } else {
             for_pops([&]<int I>() { cur[1][I] = as_calc<S>(buf[I]); });
             if constexpr (KIND != TAIL) pull(x + DEPTH, buf);
-            collide_any<S>(p, -1, x, y, cur[1], 0., m);
+            collide_bulk<S, DISC>(p, x, y, cur[1], m);
+            if constexpr (!DISC) __builtin_amdgcn_sched_barrier(0);
             for_pops([&]<int I>() { cur[1][I] = stored<S>(cur[1][I]); });
             static_for<TOP - 1>([&]<int K>() __attribute__((always_inline)) {
                 constexpr int l = K + 2;                 // level computed, column x-l+1
@@ -365,7 +376,8 @@ __device__ __forceinline__ void wpanel_in_wave(const Problem &p, const WaveGrid 
                 const T up2 = lane_up(c1[b][2]), up5 = lane_up(c2[b][1]), up6 = lane_up(cur[b][6]);
                 const T dn4 = lane_down(c1[b][4]), dn7 = lane_down(cur[b][7]), dn8 = lane_down(c2[b][2]);
                 T h[9] = {c1[b][0], c2[b][0], up2, cur[b][3], dn4, up5, up6, dn7, dn8};
-                collide_any<S>(p, -1, x - l + 1, y, h, 0., m + l - 1);
+                collide_bulk<S, DISC>(p, x - l + 1, y, h, m + l - 1);
+                if constexpr (!DISC) __builtin_amdgcn_sched_barrier(0);
                 if constexpr (l < LEVELS) {
                     for_pops([&]<int I>() { cur[l][I] = stored<S>(h[I]); });
                 } else {
@@ -408,6 +420,32 @@ __device__ __forceinline__ void wpanel_in_wave(const Problem &p, const WaveGrid 
     static_for<DEPTH>([&]<int D>() __attribute__((always_inline)) {
         iter.template operator()<TAIL, LEVELS>(x + D, f[(2 * H + D) % DEPTH]);
     });
+}
+
+// The march of interior wave t, with the bounce-back test only in waves
+// whose nodes (halo rows and columns included) can reach the disc: the
+// others, nearly all, collide without a per-lane branch (one basic block per
+// column, no merge copies; on 16384 x 4096 with the shared-denominator
+// division of macros() the six-step FP64 march runs 3.00 -> 2.73 ms per
+// launch, the four-step 2.31 -> 2.25, FP32 six-step 3.0 -> 2.9 ms).  The
+// test is wave-uniform, and both forms issue the same barriers, so a
+// workgroup's waves stay in lockstep.
+template <typename S, int DEPTH, int LEVELS>
+__device__ __forceinline__ void wpanel_in_wave(const Problem &p, const WaveGrid &g, const int t,
+                                               uint32_t dstep, const S *__restrict__ P,
+                                               S *__restrict__ Q) {
+    constexpr int H = LEVELS - 1;
+    if (t >= g.nin) return;
+    const int ck = g.c_lo + t / g.nbl;
+    const int Y0 = g.r0 + (t % g.nbl) * (64 - 2 * H);
+    const int X1 = p.lay.lx + 1 - (g.ncx - 1 - ck) * g.cx;
+    const int X0 = X1 - g.cx;
+    // FP16 keeps the branchy form everywhere: its split marches measured
+    // slower (two-step -2%, four-step -16%, six-step -9%)
+    if (sizeof(S) == 2 || box_meets_disc(p, X0 - H - DEPTH, X1 + H, Y0 - H, Y0 - H + 63))
+        wpanel_in_march<S, DEPTH, LEVELS, true>(p, g, t, dstep, P, Q);
+    else
+        wpanel_in_march<S, DEPTH, LEVELS, false>(p, g, t, dstep, P, Q);
 }
 
 template <typename S, int DEPTH, int LEVELS>

@@ -79,6 +79,31 @@ __device__ __forceinline__ void collide_any(const Problem &p, int s, int x, int 
     }
 }
 
+// Some node of columns [xa, xb] x rows [ya, yb] lies in the disc (exact:
+// the box's point nearest the centre is tested).
+__host__ __device__ inline bool box_meets_disc(const Problem &p, int xa, int xb, int ya, int yb) {
+    if (p.r2 < 0) return false;
+    const int64_t dx = p.ox < xa ? xa - p.ox : (p.ox > xb ? p.ox - xb : 0);
+    const int64_t dy = p.oy < ya ? ya - p.oy : (p.oy > yb ? p.oy - yb : 0);
+    return dx * dx + dy * dy <= p.r2;
+}
+
+// collide_any of an interior node (s = -1); DISC = false when the caller has
+// established that the node lies outside the disc, which leaves the bulk
+// collision alone, with no per-lane branch.
+template <typename S, bool DISC>
+__device__ __forceinline__ void collide_bulk(const Problem &p, int x, int y,
+                                             typename Calc<S>::T h[9], uint32_t step) {
+    if constexpr (DISC) {
+        collide_any<S>(p, -1, x, y, h, 0., step);
+    } else if constexpr (sizeof(S) == 8) {
+        bgk(h, p.omega);
+    } else {
+        const SrSeed hs = sizeof(S) == 4 ? sr_seed(p.lay.key(x, y), step) : SrSeed{0u, 0u};
+        bgk_lp<S, true>(h, p.omega, hs);
+    }
+}
+
 // a computed value as the storage format holds it
 template <typename S>
 __device__ __forceinline__ S to_storage(typename Calc<S>::T v) {
