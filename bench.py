@@ -200,7 +200,7 @@ def batching(args):
         with Batch(chans, storage) as b:
             b.init_equilibrium()
             if args.batch_tune == "on":
-                tune = autotune(b, steps=10, graph_steps=10, use_cache=False)
+                tune = autotune(b, steps=40, graph_steps=10, use_cache=False)
             else:
                 b.set_launch(0, 0, 10)
                 if args.batch_tune == "panel":

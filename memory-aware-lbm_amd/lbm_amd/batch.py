@@ -208,12 +208,13 @@ def batch_key(batch: Batch) -> str:
 
 def autotune(batch: Batch, steps: int = 20, graph_steps: int = 10,
              ty_caps=(32, 64, 128, 256), wgs=(2, 4, 8, 16), tile_cols=None,
-             panel_cols=(6, 8), cache_dir: Optional[str] = TUNE_DIR, use_cache: bool = True) -> dict:
+             panel_cols=(4, 6, 8), cache_dir: Optional[str] = TUNE_DIR,
+             use_cache: bool = True, repeats: int = 2) -> dict:
     """Pick the launch shape with the highest throughput on this batch.
 
-    Every candidate runs `steps` steps (after one warm-up replay) with
-    hipGraph replay; the state keeps advancing, which does not change the
-    per-step cost.  Candidates: the one-step launch over ty_caps x wgs and
+    Every candidate runs `steps` steps `repeats` times (after one warm-up
+    replay) with hipGraph replay and keeps its best time; the state keeps
+    advancing, which does not change the per-step cost.  Candidates: the one-step launch over ty_caps x wgs and
     the two-step launch over tile_cols (default: widths around three and two
     workgroups per CU for the storage format; skipped when a channel is
     narrower than four columns), and the two-step register march over
@@ -234,7 +235,7 @@ def autotune(batch: Batch, steps: int = 20, graph_steps: int = 10,
 
     def timed(**shape):
         batch.time_steps(graph_steps)  # capture + warm
-        ms = batch.time_steps(steps)
+        ms = min(batch.time_steps(steps) for _ in range(max(1, repeats)))
         results.append(dict(shape, glups=cells * steps / (ms * 1e-3) / 1e9))
 
     batch.set_depth(1)

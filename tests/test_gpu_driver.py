@@ -38,6 +38,25 @@ def test_driver_matches_oracle(orc, tmp_path, variant, family):
     assert np.array_equal(tmp[1:-1, 1:-1], tref[1:-1, 1:-1]), out
 
 
+@pytest.mark.parametrize("variant", ["origin", "tight_block", "panel"])
+@pytest.mark.parametrize("family", ["regularized", "zouhe"])
+def test_driver_matches_reference_sources(tmp_path, variant, family):
+    """The same command line against the reference's own lb.c / boundaries.c
+    (built unmodified into the harness, oracle/_ref): the reference's time
+    loop of unsteady.c:424-531 as the harness plays it, its sweeps and node
+    operators, not the restatement."""
+    from oracle_lib import reference
+    ref = reference()
+    if not ref.available:
+        pytest.skip("reference harness not built (oracle/_ref)")
+    lx, ly, steps = 96, 40, 30
+    lat, tmp, out = run_driver(lx, ly, steps, variant, tmp_path, family=family)
+    ch = Channel.geometry(lx, ly, "A", bc_family=1 if family == "zouhe" else 0)
+    _, lref, tref, _ = ref.run(ch, steps, variant)
+    assert np.array_equal(lat[1:-1, 1:-1], lref[1:-1, 1:-1]), out
+    assert np.array_equal(tmp[1:-1, 1:-1], tref[1:-1, 1:-1]), out
+
+
 def test_driver_savevel_format(orc, tmp_path):
     lx, ly = 40, 20
     path = tmp_path / "vel.dat"

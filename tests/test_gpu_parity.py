@@ -52,6 +52,43 @@ def test_node_operator_kat(which):
     assert np.array_equal(got, want), str(kat["bc_names"][which])
 
 
+def test_bgk_velocity_division_is_ieee(orc):
+    """The device forms ux and uy with one reciprocal of rho shared by both
+    quotients; each must still be the correctly rounded IEEE quotient the
+    reference divides out (lb.c:134-141).  BGK outputs of the oracle (C
+    division) and the device must agree bit for bit on states built to probe
+    the division: zero and one-ulp numerators of both signs, rho from 1e-3 to
+    1e3, populations scaled down to 2^-880 and up to 2^880, and large
+    velocities."""
+    rng = np.random.default_rng(7)
+    base = rng.uniform(0.01, 0.3, size=(600, 9))
+    cases = [base]
+    z = base.copy()
+    z[:, 3] = z[:, 1]; z[:, 6] = z[:, 5]; z[:, 7] = z[:, 8]      # ux numerator 0
+    cases.append(z)
+    z2 = z.copy()
+    z2[:, 4] = z2[:, 2]; z2[:, 7] = z2[:, 5]; z2[:, 8] = z2[:, 6]  # both numerators 0
+    z2[:, 3] = z2[:, 1]
+    cases.append(z2)
+    for sgn in (1.0, -1.0):                                       # one-ulp numerators
+        u = z2.copy()
+        u[:, 1] = np.nextafter(u[:, 3], u[:, 3] + sgn)
+        u[:, 2] = np.nextafter(u[:, 4], u[:, 4] - sgn)
+        cases.append(u)
+    for scale in (1e-3, 1e-1, 1e1, 1e3, 2.0 ** -880, 2.0 ** 880):
+        cases.append(base[:200] * scale)
+    fast = base[:300].copy()
+    fast[:, [1, 5, 8]] *= 3.0                                      # strong x flow
+    fast[:, [4, 7]] *= 0.2
+    cases.append(fast)
+    f = np.concatenate(cases)
+    for omega in (0.6, 1.0, 1.9):
+        got = L.apply_node_op(17, f, np.zeros(len(f)), np.zeros(len(f)), omega)
+        want = np.stack([orc.bgk(v, omega) for v in f])
+        bad = np.flatnonzero(~np.all(got.view(np.int64) == want.view(np.int64), axis=1))
+        assert bad.size == 0, f"omega {omega}: {bad.size} nodes differ, first {f[bad[0]]}"
+
+
 @pytest.mark.parametrize("name", ["traj_32x16_reg.npz", "traj_32x16_zh.npz"])
 @pytest.mark.parametrize("variant", ["origin", "combine", "tight", "tight_block", "panel"])
 def test_trajectory_golden(name, variant):
