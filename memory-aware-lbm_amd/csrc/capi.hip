@@ -153,8 +153,9 @@ void probe_done(lbm_ctx *c, KernelProbe *pr, int depth) {
 hipError_t run_collide(lbm_ctx *c, bool pull, int from, int to) {
     KernelProbe *pr = probe_for(c);
     if (pr) {
-        std::snprintf(pr->name, sizeof pr->name, "k_collide<%s, %s>", storage_name(c->storage),
-                      pull ? "true" : "false");
+        std::snprintf(pr->name, sizeof pr->name, "k_collide<%s, %s, %d>",
+                      storage_name(c->storage), pull ? "true" : "false",
+                      one_step_rows(c->prob.lay, c->storage));
         pr->nodes = (double)c->prob.lay.lx * c->prob.lay.ly;
         hipError_t e0 = hipEventRecord(pr->t0, c->stream);
         if (e0 != hipSuccess) return e0;

@@ -17,12 +17,12 @@
 //    perimeter included, with no branch.
 //
 // Kernels:
-//  k_collide<double, false>  streamed -> collided: reads L locally, collides,
+//  k_collide<double, false, R>  streamed -> collided: reads L locally, collides,
 //                    writes the other buffer (one step's collision).
-//  k_collide<double, true>   collided -> collided: pull gather + collide (one
+//  k_collide<double, true, R>   collided -> collided: pull gather + collide (one
 //                    fused step), the combine/tight hot kernel.
-//  k_stream<double>  collided -> streamed: pull gather only.
-//  (these three: rows.hpp, 16 bytes of a column per lane and population)
+//  k_stream<double, R>  collided -> streamed: pull gather only.
+//  (these three: rows.hpp, R rows of a column per lane and population)
 //  k_init            equilibrium initial condition.
 //  k_macros / k_diag moments and reductions of the post-stream state.
 //  k_periodic        the periodic ghost copy on a streamed state (any storage).
@@ -105,20 +105,17 @@ static dim3 node_grid(const Layout &L, int &threads) {
 
 hipError_t launch_collide_f64(const Problem &p, bool pull, const double *src, double *dst,
                               hipStream_t st) {
-    int t;
-    dim3 g = rows_grid<double>(p.lay, t);
-    if (pull) k_collide<double, true><<<g, t, 0, st>>>(p, src, dst);
-    else k_collide<double, false><<<g, t, 0, st>>>(p, src, dst);
-    return hipGetLastError();
+    return launch_rows<double>(pull ? 1 : 0, p, src, dst, st);
 }
 
 hipError_t launch_stream_f64(const Layout &L, const double *src, double *dst, hipStream_t st) {
     Problem q{};
     q.lay = L;
-    int t;
-    dim3 g = rows_grid<double>(L, t);
-    k_stream<double><<<g, t, 0, st>>>(q, src, dst);
-    return hipGetLastError();
+    return launch_rows<double>(2, q, src, dst, st);
+}
+
+int one_step_rows(const Layout &L, int storage) {
+    return rows_per_lane(L, storage == 0 ? 8 : storage == 1 ? 4 : 2);
 }
 
 hipError_t launch_init_f64(const Layout &L, const double *init_ux, double *dst, hipStream_t st) {

@@ -71,21 +71,14 @@ static dim3 grid_lp(const Layout &L, int &threads) {
 
 template <typename S>
 static hipError_t collide_t(const Problem &p, bool pull, const void *src, void *dst, hipStream_t st) {
-    int t;
-    dim3 g = rows_grid<S>(p.lay, t);
-    if (pull) k_collide<S, true><<<g, t, 0, st>>>(p, (const S *)src, (S *)dst);
-    else k_collide<S, false><<<g, t, 0, st>>>(p, (const S *)src, (S *)dst);
-    return hipGetLastError();
+    return launch_rows<S>(pull ? 1 : 0, p, (const S *)src, (S *)dst, st);
 }
 
 template <typename S>
 static hipError_t stream_t(const Layout &L, const void *src, void *dst, hipStream_t st) {
     Problem q{};
     q.lay = L;
-    int t;
-    dim3 g = rows_grid<S>(L, t);
-    k_stream<S><<<g, t, 0, st>>>(q, (const S *)src, (S *)dst);
-    return hipGetLastError();
+    return launch_rows<S>(2, q, (const S *)src, (S *)dst, st);
 }
 
 hipError_t launch_collide_lp(int storage, const Problem &p, bool pull, const void *src, void *dst,

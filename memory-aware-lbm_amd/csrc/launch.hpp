@@ -22,6 +22,9 @@ struct KernelProbe {
 hipError_t launch_collide_f64(const Problem &p, bool pull, const double *src, double *dst,
                               hipStream_t st);
 hipError_t launch_stream_f64(const Layout &L, const double *src, double *dst, hipStream_t st);
+// rows per lane of the one-step kernels on this lattice (storage 0 / 1 / 2:
+// FP64 / FP32 / FP16), the last template argument of their names
+int one_step_rows(const Layout &L, int storage);
 hipError_t launch_init_f64(const Layout &L, const double *init_ux, double *dst, hipStream_t st);
 // moments of columns x0 .. x0 + nx - 1 (x-major output from column x0)
 hipError_t launch_macros_f64(const Layout &L, int x0, int nx, bool pull, const double *src,

@@ -114,18 +114,19 @@ class SolveServer:
     # size and format: SWEEPS lists (minimum cells, variant) in increasing
     # size.  Measured on MI355X, one solve alone, graphs of 50 steps, GLUPS
     # fused / tight_block / panel / panel4 / panel6:
-    #   FP64 512 x 128 10.9 / 9.6 / 4.7 / 4.7 / 4.7, 1024 x 256 22.5 / 23.5 /
-    #   17.4 / 13.5 / 13.7, 1536 x 384 24.8 / 34.6 / 29.7 / 25.1 / 26.2,
-    #   2048 x 512 32.1 / 41.2 / 44.3 / 38.3 / 39.7, 3072 x 768 35.8 / 49.5 /
-    #   48.6 / 55.1 / 62.5, 4096 x 1024 37.1 / 53.8 / 49.8 / 68.3 / 78.0,
-    #   8192 x 2048 36.8 / 52.6 / 59.3 / 98.6 / 111.4;
-    #   FP32 (no panel6) 1024 x 256 29.2 / 16.2 / 15.8 / 10.8, 1536 x 384
-    #   34.3 / 31.1 / 33.9 / 24.4, 2048 x 512 47.8 / 41.0 / 50.8 / 37.0,
-    #   3072 x 768 61.1 / 59.4 / 75.9 / 63.0, 4096 x 1024 65.7 / 70.9 / 87.0 /
-    #   78.4, 8192 x 2048 66.1 / 76.3 / 105.1 / 119.1.
-    SWEEPS = {"f64": ((0, "combine"), (400_000, "tight_block"), (1_000_000, "panel"),
+    #   FP64 512 x 128 9.8 / 9.8 / 4.7 / 4.8 / 4.8, 768 x 192 18.7 / 14.2 /
+    #   10.7 / 8.1 / 8.1, 1024 x 256 23.1 / 24.3 / 17.7 / 13.6 / 14.0,
+    #   1536 x 384 27.0 / 37.3 / 31.8 / 26.5 / 28.2, 2048 x 512 34.6 / 43.6 /
+    #   45.4 / 41.1 / 43.4, 3072 x 768 38.9 / 54.0 / 51.1 / 60.3 / 68.3,
+    #   4096 x 1024 38.7 / 59.7 / 56.9 / 75.6 / 85.6, 8192 x 2048 40.7 / 63.6 /
+    #   62.1 / 108.9 / 126.6;
+    #   FP32 512 x 128 8.1 / 5.4 / 4.3 / 4.2 / 4.2, 1024 x 256 26.4 / 16.6 /
+    #   16.1 / 10.8 / 11.4, 1536 x 384 33.1 / 31.9 / 34.7 / 24.6 / 25.2,
+    #   2048 x 512 46.8 / 42.1 / 51.9 / 36.9 / 35.6, 4096 x 1024 68.7 / 73.8 /
+    #   90.4 / 79.3 / 71.5, 8192 x 2048 77.5 / 88.2 / 120.6 / 122.8 / 118.5.
+    SWEEPS = {"f64": ((0, "combine"), (200_000, "tight_block"), (1_000_000, "panel"),
                       (2_000_000, "panel6")),
-              "f32": ((0, "combine"), (800_000, "panel"), (8_000_000, "panel4")),
+              "f32": ((0, "combine"), (500_000, "panel"), (8_000_000, "panel4")),
               "f16": ((0, "combine"), (800_000, "panel"), (8_000_000, "panel4"))}
 
     def __init__(self, hbm_budget: Optional[int] = None, max_active: int = 8,
