@@ -252,21 +252,27 @@ template <int N, typename F> __device__ __forceinline__ void static_for(F &&f) {
 // WPG waves per workgroup take consecutive bands and march in lockstep (one
 // barrier per column), so the workgroup's loads of a column arrive at the
 // memory together as one contiguous run per population: eight waves.  The
-// six-step FP64 march with two columns in flight (211 VGPRs, two waves per
+// six-step FP64 march (205 VGPRs with two columns in flight, two waves per
 // SIMD) takes four: an eight-wave workgroup needs two waves on every SIMD of
-// its CU, so no perimeter wave (230 VGPRs) fits beside it, while beside a
+// its CU, so no perimeter wave (242 VGPRs) fits beside it, while beside a
 // four-wave workgroup one does (16384 x 4096: 129.1-129.6 GLUPS against
 // 116.2-116.9, and 115.0 / 119.2 with two / one; the four-step sweep, more
-// memory-bound, loses with four: 95 against 113).  With one column in flight (131 / 166 VGPRs, three waves per
-// SIMD) the FP64 four- and six-step marches take twelve, the whole CU (115.0
-// against 112.6 for the four-step sweep at 63 columns); so do the FP64
+// memory-bound, loses with four: 95 against 113).  The six-step march with
+// one column in flight takes four as well (twelve would hold it to 168
+// VGPRs, where it spills).  A six-step variant at three waves per SIMD (the
+// next level's c_x = +1 window in LDS, 164 VGPRs, one column in flight) ran
+// its interior kernel in 2.53 ms against 2.72, but then no perimeter wave
+// fits beside it until it drains, and the launch took 3.25 ms (2.70 with the
+// perimeter kernel held to 168 VGPRs and spilling).  With one column in
+// flight (126 VGPRs, three waves per SIMD) the FP64 four-step march takes
+// twelve, the whole CU (115.0 against 112.6 at 63 columns); so do the FP64
 // two-step march (63.5 against 61.6) and the FP16 four-step one (239.0
 // against 228.1).  The FP32 six-step march also takes four (134.7 against
 // 123.3; FP16 loses with four: 176.8 against 225.8).  Measured and left at
 // eight: FP32 (panel 119.9 / 120.1, panel4 129.2 / 138.5 with twelve /
 // eight), FP16 panel (212.7 / 211.7).
 template <typename S, int DEPTH, int LEVELS> constexpr int wpg_of() {
-    if constexpr (sizeof(S) >= 4 && LEVELS == 6 && DEPTH == 2) return 4;
+    if constexpr (sizeof(S) >= 4 && LEVELS == 6) return 4;
     else if constexpr (sizeof(S) == 8 && LEVELS >= 4 && DEPTH == 1) return 12;
     else if constexpr (sizeof(S) == 8 && LEVELS == 2 && DEPTH == 2) return 12;
     else if constexpr (sizeof(S) == 2 && LEVELS == 4 && DEPTH == 2) return 12;
