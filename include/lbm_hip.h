@@ -55,7 +55,10 @@ enum {
  *   PANEL6      six time steps per launch, five levels in registers (no
  *               counterpart in the reference either).  Uses a third and a
  *               fourth population buffer; lattices too small for it run as
- *               PANEL4.
+ *               PANEL4.  A remainder of 2 or 3 steps after its six-step
+ *               launches is taken as one eight-step launch in place of the
+ *               last six (seven levels in registers, a fifth buffer), or as
+ *               two four-step launches where the interior is too small.
  * All of them give bit-identical FP64 results, as the reference's serial
  * sweeps do. */
 typedef enum {
@@ -258,9 +261,10 @@ size_t lbm_hip_footprint(const lbm_ctx *ctx);
  * population buffers every variant uses. */
 size_t lbm_hip_footprint_of(const lbm_problem *prob);
 /* The same for stepping it with `variant`: LBM_PANEL4 adds the one further
- * population buffer it allocates on first use, LBM_PANEL6 two (an upper
+ * population buffer it allocates on first use, LBM_PANEL6 three (an upper
  * bound: a lattice too small for the deep march runs the shallower sweep and
- * allocates none).  No counterpart in the reference (admission support). */
+ * allocates none, and the third only comes with an eight-step launch).  No
+ * counterpart in the reference (admission support). */
 size_t lbm_hip_footprint_for(const lbm_problem *prob, int variant);
 
 /* Known-answer entry: apply one node operator of the device to n population

@@ -62,7 +62,7 @@ struct lbm_ctx {
     size_t footprint;
     int graph_steps;               // fused steps per captured graph (0: eager)
     uint32_t *d_step;              // collision counter (keys FP32 stochastic rounding)
-    void *buf3, *buf4;             // further population buffers of the four- and six-step sweeps
+    void *buf3, *buf4, *buf5;      // further population buffers of the four-, six- and eight-step launches
     hipStream_t side;              // second stream for the panel's edge waves
     hipEvent_t fork, join;
     hipGraphExec_t graph[2];       // fused steps, by starting buffer
@@ -222,10 +222,10 @@ hipError_t panel_launch(lbm_ctx *c, int from) {
     return e;
 }
 
-// one launch of `levels` (4 or 6) steps from buffer `from` into 1 - from,
-// the perimeter frame passing through buf3 (and buf4)
+// one launch of `levels` (4, 6 or 8) steps from buffer `from` into 1 - from,
+// the perimeter frame passing through buf3 (and buf4, buf5)
 hipError_t deep_launch(lbm_ctx *c, int from, int levels) {
-    void *mid[2] = {c->buf3, c->buf4};
+    void *mid[3] = {c->buf3, c->buf4, c->buf5};
     KernelProbe *pr = probe_for(c);
     hipError_t e = launch_wpanel_deep(c->storage, c->prob, c->tile_x, levels, c->buf[from], mid,
                                       c->buf[1 - from], c->stream, c->side, c->fork, c->join, pr);
@@ -350,16 +350,31 @@ int step_impl(lbm_ctx *c, int variant, int nsteps) {
         }
         // launches of `levels` steps while they fit; a six-step sweep whose
         // remainder would need a two-step launch (2 or 3 steps left) runs one
-        // six-step launch fewer and two four-step launches instead (8 = 4 +
-        // 4, 9 = 4 + 4 + 1): on 16384 x 4096 a two-step launch costs 1.0 ms
-        // per step, a four-step one 0.58 and a six-step one 0.50
+        // six-step launch fewer and one eight-step launch instead (8 = 8, 9 =
+        // 8 + 1), or two four-step launches where the lattice has no interior
+        // for eight steps (8 = 4 + 4): on 16384 x 4096 a two-step launch costs
+        // 1.0 ms per step, a four-step one 0.57, a six-step one 0.45
         int deep = (nsteps - k) / levels;
         const int rest = (nsteps - k) % levels;
-        if (levels == 6 && deep > 0 && (rest == 2 || rest == 3)) --deep;
+        int eight = 0;
+        if (levels == 6 && deep > 0 && (rest == 2 || rest == 3)) {
+            --deep;
+            if (wpanel_deep_possible(c->prob.lay, c->tile_x, c->storage, 8)) {
+                rc = extra_buffer(c, &c->buf5);
+                if (rc) return rc;
+                eight = 1;
+            }
+        }
         for (int j = 0; j < deep; ++j, k += levels) {
             HIPCHK(deep_launch(c, c->cur, levels));
             c->cur = 1 - c->cur;
             c->steps += levels;
+        }
+        if (eight) {
+            HIPCHK(deep_launch(c, c->cur, 8));
+            c->cur = 1 - c->cur;
+            c->steps += 8;
+            k += 8;
         }
         return k < nsteps ? step_impl(c, fallback, nsteps - k) : LBM_OK;
     }
@@ -464,7 +479,9 @@ size_t lbm_hip_footprint_of(const lbm_problem *p) {
 size_t lbm_hip_footprint_for(const lbm_problem *p, int variant) {
     const size_t base = lbm_hip_footprint_of(p);
     if (!base) return 0;
-    const size_t extra = variant == LBM_PANEL6 ? 2 : variant == LBM_PANEL4 ? 1 : 0;
+    // the six-step sweep's eight-step launch (for remainders of 2 or 3
+    // steps) passes its frame through a third further buffer
+    const size_t extra = variant == LBM_PANEL6 ? 3 : variant == LBM_PANEL4 ? 1 : 0;
     return base + extra * (size_t)make_layout(p->lx, p->ly).span() * elem_size(p->storage);
 }
 
@@ -567,7 +584,7 @@ int lbm_hip_destroy(lbm_ctx *c) {
         ctx_free(c, c->d_p1[s]);
     }
     for (void *p : {(void *)c->d_outlet, (void *)c->d_init, (void *)c->d_partial,
-                    (void *)c->d_out4, (void *)c->d_step, c->buf3, c->buf4,
+                    (void *)c->d_out4, (void *)c->d_step, c->buf3, c->buf4, c->buf5,
                     (void *)c->d_fields})
         ctx_free(c, p);
     if (c->pool) {

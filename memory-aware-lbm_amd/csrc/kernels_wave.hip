@@ -172,8 +172,9 @@ __device__ __forceinline__ void edge_march(const Problem &p, const WaveGrid &g, 
 }
 
 // Perimeter wave e of grid g.  GL: lanes per chunk of the thin strips
-// (g.gl; 4 for the two-step grid's two-row strips, 8 for the deep grids'
-// strips of up to six rows)
+// (g.gl; 4 for the two-step grid's two-row strips, 8 for the four- and
+// six-step grids' strips of up to six rows, 16 for the eight-step grid's
+// eight)
 template <typename S, int GL>
 __device__ __forceinline__ void wpanel_edge_wave(const Problem &p, const WaveGrid &g, int e,
                                                  uint32_t dstep, const S *__restrict__ P,
@@ -272,7 +273,7 @@ template <int N, typename F> __device__ __forceinline__ void static_for(F &&f) {
 // eight: FP32 (panel 119.9 / 120.1, panel4 129.2 / 138.5 with twelve /
 // eight), FP16 panel (212.7 / 211.7).
 template <typename S, int DEPTH, int LEVELS> constexpr int wpg_of() {
-    if constexpr (sizeof(S) >= 4 && LEVELS == 6) return 4;
+    if constexpr (sizeof(S) >= 4 && LEVELS >= 6) return 4;
     else if constexpr (sizeof(S) == 8 && LEVELS >= 4 && DEPTH == 1) return 12;
     else if constexpr (sizeof(S) == 8 && LEVELS == 2 && DEPTH == 2) return 12;
     else if constexpr (sizeof(S) == 2 && LEVELS == 4 && DEPTH == 2) return 12;
@@ -477,10 +478,10 @@ static WaveGrid wave_grid_max(const Layout &L, int cx, int cmax, int64_t waves) 
     }
     g.cx = cx;
     // the interior kernel's buffer resources span one chunk's columns and its
-    // 32-bit load offsets the chunk plus its halo columns (at most 2 x 5 + 2
+    // 32-bit load offsets the chunk plus its halo columns (at most 2 x 7 + 2
     // more): both stay below the drop offset (2 GiB) at up to 8 bytes per
     // element (lattices too tall for 3 columns: wave_grid_fits)
-    while (g.cx > 3 && (int64_t)(g.cx + 12) * L.pitch * 8 >= (int64_t)DROP) g.cx /= 2;
+    while (g.cx > 3 && (int64_t)(g.cx + 16) * L.pitch * 8 >= (int64_t)DROP) g.cx /= 2;
     g.nbands = (L.ly + WAVE_ROWS - 1) / WAVE_ROWS;
     g.ncx = (L.lx + g.cx - 1) / g.cx;
     g.ntasks = g.nbands * g.ncx;
@@ -538,7 +539,7 @@ static WaveGrid make_thin(const Layout &L, WaveGrid g, int H, int min_rows) {
     g.b_lo = g.b_hi = 0;   // the thin frame has no (band, chunk) rectangle
     g.nbl = (g.r1 - g.r0 + rows - 1) / rows;
     g.nin = g.nbl * (g.c_hi - g.c_lo);
-    g.gl = H + 3 <= 4 ? 4 : 8;   // H + 1 strip rows and two halo rows per chunk
+    g.gl = H + 3 <= 4 ? 4 : H + 3 <= 8 ? 8 : 16;   // H + 1 strip rows and two halo rows per chunk
     const int cpw = 64 / g.gl;
     g.nthin = 2 * ((g.ncx + cpw - 1) / cpw);
     g.nedge = g.nthin + (g.r1 - g.r0 + WAVE_ROWS - 1) / WAVE_ROWS * (g.c_lo + g.ncx - g.c_hi);
@@ -552,7 +553,7 @@ WaveGrid wave_grid(const Layout &L, int cx) {
     return make_thin(L, wave_grid_max(L, cx, 16, 16384), 1, WAVE_ROWS);
 }
 
-// The grid of the deep sweeps (LEVELS = 4 or 6 steps per launch, H = LEVELS-1
+// The grid of the deep sweeps (LEVELS = 4, 6 or 8 steps per launch, H = LEVELS-1
 // halo rows and columns): the interior chunks of the two-step grid, shrunk
 // until their H-column halo stays off columns 1 and lx, over rows r0 = H + 2
 // .. r1 = ly - H, the rows whose H-row halo stays within rows 2 .. ly - 1,
@@ -582,17 +583,17 @@ WaveGrid wave_grid(const Layout &L, int cx) {
 static WaveGrid wave_grid_deep(const Layout &L, int cx, int storage, int levels) {
     const int64_t bands = (L.ly + WAVE_ROWS - 1) / WAVE_ROWS;
     const bool largest = bands * L.lx >= 128 * 6144;
-    const int cmax = !largest ? 24 : levels == 6 ? 64 : storage != 2 ? 40 : 24;
+    const int cmax = !largest ? 24 : levels >= 6 ? 64 : storage != 2 ? 40 : 24;
     if (cx < 3 && largest && levels == 4 && storage == 0) cx = 63;
     // at least two bands of interior rows: they hold the rings of the
     // perimeter passes, at most LEVELS - 2 nodes thick, many times over
     return make_thin(L, wave_grid_max(L, cx, cmax, 6144), levels - 1, 2 * WAVE_ROWS);
 }
 
-bool wave_grid_fits(const Layout &L) { return (int64_t)(3 + 12) * L.pitch * 8 < (int64_t)DROP; }
+bool wave_grid_fits(const Layout &L) { return (int64_t)(3 + 16) * L.pitch * 8 < (int64_t)DROP; }
 
 bool wpanel_deep_possible(const Layout &L, int cx, int storage, int levels) {
-    if (levels != 4 && levels != 6) return false;
+    if (levels != 4 && levels != 6 && levels != 8) return false;
     if (!wave_grid_fits(L)) return false;
     WaveGrid g = wave_grid_deep(L, cx, storage, levels);
     // an interior of at least 124 rows and two chunks
@@ -780,7 +781,10 @@ hipError_t launch_wpanel_deep(int storage, const Problem &p, int cx, int levels,
     const Layout &L = p.lay;
     const WaveGrid g = wave_grid_deep(L, cx, storage, levels);
     const int nmid = levels / 2 - 1;
-    const int depth = g.cx % 2 == 0 ? 2 : 1;
+    // two columns of loads in flight when the chunk width allows, except for
+    // the eight-step FP64 march (two columns in flight need 265 registers,
+    // one wave per SIMD; one column 252, two waves)
+    const int depth = g.cx % 2 == 0 && !(levels == 8 && storage == 0) ? 2 : 1;
     Ring ring;
     ring.xa = L.lx + 1 - (g.ncx - g.c_lo) * g.cx;
     ring.xb = L.lx + 1 - (g.ncx - g.c_hi) * g.cx;
@@ -801,7 +805,8 @@ hipError_t launch_wpanel_deep(int storage, const Problem &p, int cx, int levels,
         for (int j = 0; j <= nmid; ++j) {
             const T *src = (const T *)(j == 0 ? P : mid[j - 1]);
             T *dst = (T *)(j == nmid ? Q : mid[j]);
-            k_wpanel_edge<T, 8><<<g.nedge, 64, 0, side>>>(p, g, 2u * j, src, dst);
+            if (g.gl == 16) k_wpanel_edge<T, 16><<<g.nedge, 64, 0, side>>>(p, g, 2u * j, src, dst);
+            else k_wpanel_edge<T, 8><<<g.nedge, 64, 0, side>>>(p, g, 2u * j, src, dst);
             if (j < nmid) {
                 Ring r = ring;
                 r.t = 2 + 2 * (nmid - 1 - j);
@@ -815,9 +820,12 @@ hipError_t launch_wpanel_deep(int storage, const Problem &p, int cx, int levels,
         if (levels == 4) {
             if (depth == 2) launch_in<T, 2, 4>(p, g, 0u, src, dst, st, probe);
             else launch_in<T, 1, 4>(p, g, 0u, src, dst, st, probe);
-        } else {
+        } else if (levels == 6) {
             if (depth == 2) launch_in<T, 2, 6>(p, g, 0u, src, dst, st, probe);
             else launch_in<T, 1, 6>(p, g, 0u, src, dst, st, probe);
+        } else {
+            if (depth == 2) launch_in<T, 2, 8>(p, g, 0u, src, dst, st, probe);
+            else launch_in<T, 1, 8>(p, g, 0u, src, dst, st, probe);
         }
     };
     if (storage == 0) go(double{});

@@ -106,7 +106,7 @@ bool wave_grid_fits(const Layout &L);
 hipError_t launch_wpanel(int storage, const Problem &p, const WaveGrid &g, const void *P, void *Q,
                          hipStream_t st, hipStream_t side, hipEvent_t fork, hipEvent_t join,
                          KernelProbe *probe = nullptr);
-// LEVELS = 4 or 6 steps per launch: the interior waves march LEVELS levels
+// LEVELS = 4, 6 or 8 steps per launch: the interior waves march LEVELS levels
 // in registers (P -> Q); the perimeter frame runs LEVELS/2 two-step launches
 // through LEVELS/2 - 1 further buffers mid[] (P -> mid[0] over the frame
 // widened by LEVELS/2 - 1 bands / chunks, ..., the last one into Q).

@@ -66,8 +66,10 @@ def test_footprint_without_allocating():
 
 
 def test_footprint_of_a_sweep_counts_its_buffers():
-    """Admission support: the four- and six-step sweeps allocate one and two
-    further population buffers of 9 planes of (lx + 2) x pitch elements."""
+    """Admission support: the four- and six-step sweeps allocate one and
+    three further population buffers of 9 planes of (lx + 2) x pitch elements
+    (the six-step sweep's eight-step launch, for remainders of 2 or 3 steps,
+    takes the third)."""
     import lbm_amd
     for lx, ly, st, es in ((2048, 1024, "f64", 8), (300, 77, "f32", 4), (64, 5, "f16", 2)):
         ch = lbm_amd.Channel.geometry(lx, ly, "B")
@@ -77,7 +79,7 @@ def test_footprint_of_a_sweep_counts_its_buffers():
         assert lbm_amd.footprint_of(ch, st, "combine") == base
         assert lbm_amd.footprint_of(ch, st, "panel") == base
         assert lbm_amd.footprint_of(ch, st, "panel4") == base + buf
-        assert lbm_amd.footprint_of(ch, st, "panel6") == base + 2 * buf
+        assert lbm_amd.footprint_of(ch, st, "panel6") == base + 3 * buf
 
 
 def test_unbound_sweep_aborts_without_gpu():

@@ -4,8 +4,8 @@ reference's boundary operators and outlet (unsteady.c:84-109, 162-214).
 
 Every sweep the bench times runs 24 steps from the initial equilibrium, in
 three calls of 13, 8 and 3 steps: after the first collision the deep sweeps
-launch at every depth they have (panel6: two six-step launches, then two
-four-step launches in place of a six- and a two-step one, then a two-step
+launch at every depth they have (panel6: two six-step launches, then one
+eight-step launch in place of a six- and a two-step one, then a two-step
 launch and a single step; panel4: three four-step launches, then two, then a
 two-step launch and a single step).  Each result is compared with the
 oracle's (the reference's origin order, lb.c:172-341, restated in C) on all
@@ -52,7 +52,7 @@ def test_bench_lattice_matches_oracle(reference, variant):
         lo, to, po = lat.download()
     depths = {r["depth"] for r in recs}
     want = {"origin": {1}, "combine": {1}, "tight": {1}, "tight_block": {1, 2},
-            "panel": {1, 2}, "panel4": {1, 2, 4}, "panel6": {1, 2, 4, 6}}[variant]
+            "panel": {1, 2}, "panel4": {1, 2, 4}, "panel6": {1, 2, 6, 8}}[variant]
     assert depths == want, recs
     ok_l = same(lo, lref)
     ok_t = same(to, tref)

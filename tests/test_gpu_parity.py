@@ -279,9 +279,10 @@ def test_panel6_against_oracle(orc, geom):
     """Six steps per launch (five levels in registers, the perimeter frame
     through two further buffers) against the oracle, bit for bit, for step
     counts that leave every remainder (a four-step, a two-step launch, a
-    single fused step; 9 and 10 steps run two four-step launches in place of
-    a six- and a two-step one) and for chunk widths with one and two columns
-    in flight, up to the 64 columns the largest lattices use."""
+    single fused step; 9 and 10 steps run one eight-step launch, seven levels
+    in registers and the frame through three further buffers, in place of a
+    six- and a two-step one) and for chunk widths with one and two columns in
+    flight, up to the 64 columns the largest lattices use."""
     ch = OChannel.geometry(600, 400, geom)
     for steps in (6, 7, 8, 9, 10, 13, 19):
         _, lref, tref, pref = orc.run(ch, steps)
@@ -299,10 +300,12 @@ def test_panel6_against_oracle(orc, geom):
 
 def test_panel6_launch_plan():
     """Twenty steps of the six-step sweep from a collided state run as
-    6 + 6 + 4 + 4 (a two-step launch costs twice a four-step one per step),
-    and 21 as 6 + 6 + 4 + 4 + 1."""
-    ch = L.Channel(600, 400, 120, 200, 40)
-    for n, want in ((20, {6: 2, 4: 2}), (21, {6: 2, 4: 2, 1: 1}), (18, {6: 3}), (22, {6: 3, 4: 1})):
+    6 + 6 + 8 (a two-step launch costs twice a four-step one per step), and
+    21 as 6 + 6 + 8 + 1; on a lattice with an interior deep enough for six
+    levels but not for eight (300 x 136), 20 run as 6 + 6 + 4 + 4."""
+    big, small = L.Channel(600, 400, 120, 200, 40), L.Channel(300, 136, 60, 68, 13)
+    for ch, n, want in ((big, 20, {6: 2, 8: 1}), (big, 21, {6: 2, 8: 1, 1: 1}),
+                        (big, 18, {6: 3}), (big, 22, {6: 3, 4: 1}), (small, 20, {6: 2, 4: 2})):
         with L.Lattice(ch) as lat:
             lat.init_equilibrium()
             lat.step(1, "panel6")
