@@ -295,12 +295,18 @@ template <typename S, int DEPTH, int LEVELS> constexpr int wpg_of() {
 // DISC: the wave's nodes may reach the disc (per-lane bounce-back branch).
 // Without that branch an iteration is one basic block, and the scheduler
 // interleaves its collisions so freely that the six-step FP64 march needs
-// 256 registers and spills (1.3x slower); a scheduling fence after each
-// collision keeps the branchy form's register use (205 VGPRs, two waves per
-// SIMD) and still drops its branches and the copies at their merges.
+// 256 registers and spills (1.3x slower); scheduling fences between the
+// collisions (fence_after) keep the branchy form's register use (205 VGPRs,
+// two waves per SIMD) and still drop its branches and the copies at their
+// merges.
 // Fences that let memory or scalar instructions cross measured the same;
 // unrolling the march by six columns (no window copies at all, 13% fewer
 // instructions per column) 1.5% faster for FP64 and 15% slower for FP32.
+
+// scheduling fence after the collisions of levels 1, 4, 7 (every collision
+// fenced: two-step march 64.5 GLUPS, four-step 117.5; fenced after levels 1,
+// 4, 7: 68.3 and 119.3; six-step unchanged at 205 VGPRs either way)
+constexpr bool fence_after(int level) { return level % 3 == 1; }
 
 template <typename S, int DEPTH, int LEVELS, bool DISC>
 __device__ __forceinline__ void wpanel_in_march(const Problem &p, const WaveGrid &g, const int t,
@@ -375,7 +381,7 @@ __device__ __forceinline__ void wpanel_in_march(const Problem &p, const WaveGrid
             for_pops([&]<int I>() { cur[1][I] = as_calc<S>(buf[I]); });
             if constexpr (KIND != TAIL) pull(x + DEPTH, buf);
             collide_bulk<S, DISC>(p, x, y, cur[1], m);
-            if constexpr (!DISC) __builtin_amdgcn_sched_barrier(0);
+            if constexpr (!DISC && fence_after(1)) __builtin_amdgcn_sched_barrier(0);
             for_pops([&]<int I>() { cur[1][I] = stored<S>(cur[1][I]); });
             static_for<TOP - 1>([&]<int K>() __attribute__((always_inline)) {
                 constexpr int l = K + 2;                 // level computed, column x-l+1
@@ -384,7 +390,7 @@ __device__ __forceinline__ void wpanel_in_march(const Problem &p, const WaveGrid
                 const T dn4 = lane_down(c1[b][4]), dn7 = lane_down(cur[b][7]), dn8 = lane_down(c2[b][2]);
                 T h[9] = {c1[b][0], c2[b][0], up2, cur[b][3], dn4, up5, up6, dn7, dn8};
                 collide_bulk<S, DISC>(p, x - l + 1, y, h, m + l - 1);
-                if constexpr (!DISC) __builtin_amdgcn_sched_barrier(0);
+                if constexpr (!DISC && fence_after(l)) __builtin_amdgcn_sched_barrier(0);
                 if constexpr (l < LEVELS) {
                     for_pops([&]<int I>() { cur[l][I] = stored<S>(h[I]); });
                 } else {
