@@ -89,10 +89,11 @@ def parse():
     ap.add_argument("--batch", type=int, default=512,
                     help="channels of the batching workload (0 = skip)")
     ap.add_argument("--batch-seed", type=int, default=0)
-    ap.add_argument("--batch-steps", type=int, default=100)
-    ap.add_argument("--batch-tune", default="on", choices=["on", "off", "panel"],
+    ap.add_argument("--batch-steps", type=int, default=96)
+    ap.add_argument("--batch-tune", default="on", choices=["on", "off", "panel", "panel4", "panel6"],
                     help="autotune the batch launch shape (off: the library's defaults, "
-                         "panel: the register march at its default chunk width)")
+                         "panel / panel4 / panel6: that register march at its default chunk "
+                         "width)")
     ap.add_argument("--serve", type=int, default=40,
                     help="requests of the serving workload (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrivals per second")
@@ -195,8 +196,9 @@ def summary(r, storage):
 def batching(args):
     """The batching workload: seeded uneven channels, hipGraph replay,
     autotuned launch shape (two steps per launch over LDS tiles or by the
-    register march over every channel's wave grid, or one persistent launch
-    per step); FP64, FP32 and FP16 storage."""
+    register march over every channel's wave grid, four or six by the deep
+    register marches, or one persistent launch per step); FP64, FP32 and
+    FP16 storage."""
     from lbm_amd.batch import Batch, autotune, channels_from_seed
     chans = channels_from_seed(args.batch, seed=args.batch_seed)
     out = {"channels": len(chans), "seed": args.batch_seed,
@@ -208,25 +210,30 @@ def batching(args):
             if args.batch_tune == "on":
                 tune = autotune(b, steps=40, graph_steps=10, use_cache=False)
             else:
-                b.set_launch(0, 0, 10)
-                if args.batch_tune == "panel":
-                    b.set_sweep("panel")
+                b.set_launch(0, 0, 12)
+                if args.batch_tune != "off":
+                    b.set_sweep(args.batch_tune)
                 tune = dict(b.launch)
-            b.step(10)
+            spl = b.steps_per_launch()
+            b.step(tune["graph_steps"] or 12)
             ms = b.time_steps(args.batch_steps)
             glups = b.cells * args.batch_steps / (ms * 1e-3) / 1e9
             bpu = BYTES_PER_UPDATE[storage]
-            # a two-step launch reads and writes every population once
-            ach = glups * bpu / tune["depth"]
+            # a k-step launch reads and writes every population once (with a
+            # step count that is a multiple of the graph's, every launch is k
+            # steps deep)
+            ach = glups * bpu / spl
             out[storage] = {"glups": round(glups, 4), "ms_per_step": round(ms / args.batch_steps, 5),
                             "achieved_gbs": round(ach, 1),
                             "frac": round(ach / HBM_PEAK_GBS, 4),
                             "update_equivalent_gbs": round(glups * bpu, 1),
-                            "launch": {"depth": tune["depth"], "tile_cols": tune["tile_cols"],
+                            "launch": {"depth": tune["depth"], "steps_per_launch": spl,
+                                       "tile_cols": tune["tile_cols"],
                                        "ty_cap": tune["ty_cap"], "wgs_per_cu": tune["wgs_per_cu"],
                                        "graph_steps": tune["graph_steps"],
                                        "sweep": tune.get("sweep", "tight_block"),
-                                       "chunk_cols": tune.get("chunk_cols", 0)},
+                                       "chunk_cols": tune.get("chunk_cols", 0),
+                                       "panel_cols": tune.get("panel_cols", 0)},
                             "footprint_gb": round(b.footprint() / 1e9, 3)}
     return out
 

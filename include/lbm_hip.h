@@ -320,15 +320,23 @@ int lbm_hip_batch_set_launch(lbm_batch *b, int wgs_per_cu, int ty_cap, int graph
  * (default 8), [3, 69] for FP32 / FP16 (default 14 / 10); 0 keeps it.  Odd
  * remainders and the first step after init_equilibrium run one step. */
 int lbm_hip_batch_set_depth(lbm_batch *b, int depth, int tile_cols);
-/* The sweep of the two-step launches: LBM_TIGHT_BLOCK (the LDS tiles above,
- * the default) or LBM_PANEL (every channel's waves march its columns with
- * the first step in wave64 registers, the panel sweep's own per-wave code
- * over each channel's wave grid of chunk_cols-column chunks, from flat work
- * tables: one launch for the perimeter waves of every channel, on a second
- * stream, beside one for the interior waves).  chunk_cols: even, in [4, 64]
- * (default 6; 0 keeps it).  Same bits either way.  LBM_ERR_UNSUPPORTED for
- * a channel the panel sweep refuses (lx < 4, or too tall for its 32-bit
- * offsets). */
+/* The sweep of the multi-step launches: LBM_TIGHT_BLOCK (two steps, the LDS
+ * tiles above, the default), LBM_PANEL (two steps: every channel's waves
+ * march its columns with the first step in wave64 registers, the panel
+ * sweep's own per-wave code over each channel's wave grid of
+ * chunk_cols-column chunks, from flat work tables: one launch for the
+ * perimeter waves of every channel, on a second stream, beside one for the
+ * interior waves), or LBM_PANEL4 / LBM_PANEL6 (four / six steps: the deep
+ * register marches over every channel's interior, rows and columns H + 2 ..
+ * ly - H - 1 / lx - H - 1 with H = 3 / 5, in chunk_cols-column chunks, one
+ * launch for all channels; on the second stream the frame around them in
+ * two / three two-step passes through one / two further pool-sized buffers
+ * (each pass but the last widened inwards by what the next one reads), and
+ * two-step passes of LBM_PANEL for channels too small for the deep march;
+ * remainders of two steps take LBM_PANEL).  chunk_cols: even, in [4, 64]
+ * (default 6 for LBM_PANEL, 24 for the deep marches; 0 keeps it).  Same bits
+ * every way.  LBM_ERR_UNSUPPORTED for a channel the panel sweep refuses
+ * (lx < 4, or too tall for its 32-bit offsets). */
 int lbm_hip_batch_set_sweep(lbm_batch *b, int variant, int chunk_cols);
 int lbm_hip_batch_step(lbm_batch *b, int nsteps);
 int lbm_hip_batch_sync(lbm_batch *b);
@@ -339,6 +347,9 @@ int lbm_hip_batch_download(lbm_batch *b, int c, double *lattice, double *tmp,
 int lbm_hip_batch_macros(lbm_batch *b, int c, double *rho, double *ux, double *uy);
 int64_t lbm_hip_batch_cells(const lbm_batch *b);
 int64_t lbm_hip_batch_steps_done(const lbm_batch *b);
+/* device bytes held: the two population pools, the further buffers of the
+ * deep marches once set (one for LBM_PANEL4, two once LBM_PANEL6 was), and
+ * the work tables */
 size_t lbm_hip_batch_footprint(const lbm_batch *b);
 
 /* Description of the last error on this thread. */

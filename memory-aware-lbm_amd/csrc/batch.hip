@@ -150,15 +150,28 @@ struct lbm_batch {
     int *d_tile_chan = nullptr;                     // channel of each two-step tile
     int64_t ntiles = 0;
     std::vector<int> ly_of, lx_of;
-    // two-step sweep: LDS tiles (LBM_TIGHT_BLOCK) or the register march
-    // (LBM_PANEL) with its per-channel wave grids and work tables
+    // multi-step sweep: LDS tiles (LBM_TIGHT_BLOCK, two steps), the register
+    // march (LBM_PANEL, two steps) with its per-channel wave grids and work
+    // tables, or the deep register marches (LBM_PANEL4 / LBM_PANEL6, four /
+    // six steps) with theirs beside the two-step ones (their channels without
+    // a deep interior, and remainders of two steps)
     int sweep = LBM_TIGHT_BLOCK;
-    int panel_cx = 6;                               // the register march's chunk width
+    int panel_cx = 6;                               // the two-step march's chunk width
     hipStream_t side = nullptr;                     // its perimeter waves' stream
     hipEvent_t fork = nullptr, join = nullptr;
+    std::vector<ChannelWaves> cw;                   // two-step grids, every channel
     ChannelWaves *d_cw = nullptr;
     int2 *d_edge_items = nullptr, *d_in_items = nullptr;
     int nedge_items = 0, nin_wgs = 0;
+    int deep_levels = 0;                            // 4 or 6 once a deep sweep was set
+    int deep_cx = 24;                               // the deep marches' chunk width
+    int ndeep = 0;                                  // channels with a deep grid
+    ChannelWaves *d_dcw = nullptr;                  // their deep grids
+    int2 *d_din = nullptr, *d_frame[3] = {nullptr, nullptr, nullptr};
+    int ndin_wgs = 0, nframe[3] = {0, 0, 0};
+    int2 *d_sedge = nullptr, *d_sin = nullptr;      // the other channels' two-step tables
+    int nsedge = 0, nsin_wgs = 0;
+    void *mid[2] = {nullptr, nullptr};              // the deep launches' further buffers
 };
 
 namespace {
@@ -254,8 +267,72 @@ int build_waves(lbm_batch *b) {
     }
     b->nedge_items = (int)edge.size();
     b->nin_wgs = (int)in.size();
+    b->cw = std::move(cw);
     return LBM_OK;
 }
+
+// v to a fresh device array *d (nullptr when empty), the old one freed
+template <typename T> int upload(const std::vector<T> &v, T **d) {
+    if (*d) (void)hipFree(*d);
+    *d = nullptr;
+    if (v.empty()) return LBM_OK;
+    HIPCHK(hipMalloc((void **)d, v.size() * sizeof(T)));
+    HIPCHK(hipMemcpy(*d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return LBM_OK;
+}
+
+// the deep marches' grids and work tables (deep_levels steps, deep_cx-column
+// chunks) for the channels that take them, the two-step tables (b->cw,
+// built first) of the others, and the further buffers
+int build_deep(lbm_batch *b) {
+    const int levels = b->deep_levels, nmid = levels / 2 - 1;
+    const int wpg = batch_deep_wpg(b->storage, levels), swpg = batch_wpanel_wpg(b->storage);
+    std::vector<ChannelWaves> dcw;
+    std::vector<int2> in, frame[3], sedge, sin;
+    for (int c = 0; c < b->n; ++c) {
+        const BatchEntry &e = b->ents[c];
+        ChannelWaves w;
+        std::memset(&w, 0, sizeof w);
+        w.p = e.p;
+        w.off = e.off;
+        if (batch_deep_grid(e.p.lay, b->deep_cx, levels, &w.g, &w.f)) {
+            const int d = (int)dcw.size();
+            dcw.push_back(w);
+            for (int k = 0; k < (w.g.nin + wpg - 1) / wpg; ++k) in.push_back(make_int2(d, k));
+            for (int j = 0; j <= nmid; ++j)
+                for (int k = 0; k < batch_frame_waves(w.f, j); ++k) frame[j].push_back(make_int2(d, k));
+        } else {
+            const WaveGrid &g = b->cw[c].g;
+            for (int k = 0; k < g.nedge; ++k) sedge.push_back(make_int2(c, k));
+            for (int k = 0; k < (g.nin + swpg - 1) / swpg; ++k) sin.push_back(make_int2(c, k));
+        }
+    }
+    for (const auto *v : {&in, &frame[0], &frame[1], &frame[2], &sedge, &sin})
+        if (v->size() > INT32_MAX) return fail(LBM_ERR_ARG, "batch has more than 2^31 waves");
+    int rc = upload(dcw, &b->d_dcw);
+    if (!rc) rc = upload(in, &b->d_din);
+    for (int j = 0; j < 3 && !rc; ++j) rc = upload(frame[j], &b->d_frame[j]);
+    if (!rc) rc = upload(sedge, &b->d_sedge);
+    if (!rc) rc = upload(sin, &b->d_sin);
+    for (int j = 0; j < nmid && !rc; ++j)
+        if (!b->mid[j]) {
+            hipError_t e = hipMalloc(&b->mid[j], (size_t)b->pool_elems * esize(b->storage));
+            if (e != hipSuccess) { b->mid[j] = nullptr; rc = hip_fail(e, "hipMalloc deep buffer"); }
+        }
+    if (rc) {
+        b->ndeep = b->ndin_wgs = b->nsedge = b->nsin_wgs = 0;
+        b->nframe[0] = b->nframe[1] = b->nframe[2] = 0;
+        return rc;
+    }
+    b->ndeep = (int)dcw.size();
+    b->ndin_wgs = (int)in.size();
+    for (int j = 0; j < 3; ++j) b->nframe[j] = (int)frame[j].size();
+    b->nsedge = (int)sedge.size();
+    b->nsin_wgs = (int)sin.size();
+    return LBM_OK;
+}
+
+int sweep_levels(int sweep) { return sweep == LBM_PANEL4 ? 4 : sweep == LBM_PANEL6 ? 6 : 2; }
 
 hipError_t launch_step(lbm_batch *b, bool pull) {
     int nwg = std::max(1, b->num_cu * b->wgs_per_cu);
@@ -284,10 +361,10 @@ hipError_t launch_pair(lbm_batch *b) {
     const size_t lds = batch_block_lds(b->tile_cols, b->storage);
     const void *src = b->pool[b->cur];
     void *dst = b->pool[1 - b->cur];
-    if (b->sweep == LBM_PANEL) {
-        hipError_t e = launch_batch_wpanel(b->storage, b->d_cw, b->d_edge_items, b->nedge_items,
-                                           b->d_in_items, b->nin_wgs, src, dst, b->stream,
-                                           b->side, b->fork, b->join);
+    if (b->sweep != LBM_TIGHT_BLOCK) {
+        const BatchTables t{b->d_cw, b->d_edge_items, b->nedge_items, b->d_in_items, b->nin_wgs};
+        hipError_t e = launch_batch_wpanel(b->storage, t, 0u, src, dst, b->stream, b->side,
+                                           b->fork, b->join);
         if (e == hipSuccess && b->storage != LBM_STORE_F64) e = launch_bump(b->d_step, 2, b->stream);
         return e;
     }
@@ -309,15 +386,36 @@ hipError_t launch_pair(lbm_batch *b) {
     return e == hipSuccess ? launch_bump(b->d_step, 2, b->stream) : e;
 }
 
-// steps of the next launch: two where the depth and the state allow it
+// four or six steps of every channel from the collided state
+hipError_t launch_deep(lbm_batch *b) {
+    const int levels = sweep_levels(b->sweep);
+    const BatchTables deep{b->d_dcw, nullptr, 0, b->d_din, b->ndin_wgs};
+    const BatchTables shallow{b->d_cw, b->d_sedge, b->nsedge, b->d_sin, b->nsin_wgs};
+    const int2 *const frames[3] = {b->d_frame[0], b->d_frame[1], b->d_frame[2]};
+    hipError_t e = launch_batch_wpanel_deep(b->storage, levels, deep, frames, b->nframe, shallow,
+                                            b->pool[b->cur], b->mid, b->pool[1 - b->cur],
+                                            b->stream, b->side, b->fork, b->join);
+    if (e == hipSuccess && b->storage != LBM_STORE_F64)
+        e = launch_bump(b->d_step, (uint32_t)levels, b->stream);
+    return e;
+}
+
+// steps of the next launch: the sweep's depth where the setting and the
+// state allow it, else two (the two-step form of the sweep) or one
 int launch_depth(const lbm_batch *b, int left, bool collided) {
-    return b->depth == 2 && collided && left >= 2 ? 2 : 1;
+    if (b->depth != 2 || !collided || left < 2) return 1;
+    const int levels = sweep_levels(b->sweep);
+    return left >= levels ? levels : 2;
+}
+
+hipError_t launch_k(lbm_batch *b, int k, bool collided) {
+    return k == 1 ? launch_step(b, collided) : k == 2 ? launch_pair(b) : launch_deep(b);
 }
 
 int eager_steps(lbm_batch *b, int nsteps) {
     while (nsteps > 0) {
         const int k = launch_depth(b, nsteps, b->collided != 0);
-        HIPCHK(k == 2 ? launch_pair(b) : launch_step(b, b->collided != 0));
+        HIPCHK(launch_k(b, k, b->collided != 0));
         b->cur = 1 - b->cur;
         b->collided = 1;
         b->steps += k;
@@ -336,7 +434,7 @@ int capture(lbm_batch *b, int start) {
     int flips = 0;
     for (int left = b->graph_steps; left > 0 && e == hipSuccess;) {
         const int k = launch_depth(b, left, true);
-        e = k == 2 ? launch_pair(b) : launch_step(b, true);
+        e = launch_k(b, k, true);
         b->cur = 1 - b->cur;
         ++flips;
         left -= k;
@@ -500,6 +598,10 @@ int lbm_hip_batch_destroy(lbm_batch *b) {
     if (b->d_cw) (void)hipFree(b->d_cw);
     if (b->d_edge_items) (void)hipFree(b->d_edge_items);
     if (b->d_in_items) (void)hipFree(b->d_in_items);
+    for (void *q : {(void *)b->d_dcw, (void *)b->d_din, (void *)b->d_frame[0],
+                    (void *)b->d_frame[1], (void *)b->d_frame[2], (void *)b->d_sedge,
+                    (void *)b->d_sin, b->mid[0], b->mid[1]})
+        if (q) (void)hipFree(q);
     if (b->d_step) (void)hipFree(b->d_step);
     if (b->side) (void)hipStreamSynchronize(b->side);
     if (b->fork) (void)hipEventDestroy(b->fork);
@@ -591,21 +693,48 @@ int lbm_hip_batch_set_depth(lbm_batch *b, int depth, int tile_cols) {
 }
 
 int lbm_hip_batch_set_sweep(lbm_batch *b, int variant, int chunk_cols) {
-    if (!b || (variant != LBM_TIGHT_BLOCK && variant != LBM_PANEL))
-        return fail(LBM_ERR_ARG, "a batch's two-step sweep is LBM_TIGHT_BLOCK or LBM_PANEL");
+    if (!b || (variant != LBM_TIGHT_BLOCK && variant != LBM_PANEL && variant != LBM_PANEL4 &&
+               variant != LBM_PANEL6))
+        return fail(LBM_ERR_ARG, "a batch's multi-step sweep is LBM_TIGHT_BLOCK, LBM_PANEL, "
+                                 "LBM_PANEL4 or LBM_PANEL6");
     if (chunk_cols != 0 && (chunk_cols < 4 || chunk_cols > 64 || chunk_cols % 2))
         return fail(LBM_ERR_ARG, "chunk_cols must be even, in [4, 64] (0: keep)");
-    const int cx = chunk_cols ? chunk_cols : b->panel_cx;
-    if (variant == b->sweep && cx == b->panel_cx) return LBM_OK;
+    const bool deep = variant == LBM_PANEL4 || variant == LBM_PANEL6;
+    const int levels = sweep_levels(variant);
+    const int cx = chunk_cols ? chunk_cols : deep ? b->deep_cx : b->panel_cx;
+    if (variant == b->sweep && cx == (deep ? b->deep_cx : b->panel_cx)) return LBM_OK;
     HIPCHK(hipStreamSynchronize(b->stream));
     drop_graphs(b);
-    if (variant == LBM_PANEL && (!b->d_cw || cx != b->panel_cx)) {
+    if (variant != LBM_TIGHT_BLOCK && (!b->d_cw || (!deep && cx != b->panel_cx))) {
         const int old = b->panel_cx;
-        b->panel_cx = cx;
+        if (!deep) b->panel_cx = cx;
         const int rc = build_waves(b);
-        if (rc) { b->panel_cx = old; return rc; }
+        if (rc) {
+            b->panel_cx = old;
+            if (rc != LBM_ERR_UNSUPPORTED && rc != LBM_ERR_ARG) {
+                // a device allocation failed part-way: the two-step tables are gone
+                if (b->d_cw) (void)hipFree(b->d_cw);
+                b->d_cw = nullptr;
+                b->nedge_items = b->nin_wgs = 0;
+                b->deep_levels = 0;
+                b->sweep = LBM_TIGHT_BLOCK;
+            }
+            return rc;
+        }
+        b->deep_levels = 0;   // deep tables, if any, list the old two-step grids
     }
-    b->panel_cx = cx;
+    if (deep && (!b->d_dcw || cx != b->deep_cx || levels != b->deep_levels)) {
+        b->deep_cx = cx;
+        b->deep_levels = levels;
+        const int rc = build_deep(b);
+        if (rc) {
+            // the deep tables are gone: fall back to the two-step march
+            b->deep_levels = 0;
+            b->sweep = LBM_PANEL;
+            return rc;
+        }
+    }
+    if (!deep) b->panel_cx = cx;
     b->sweep = variant;
     return LBM_OK;
 }
@@ -715,7 +844,8 @@ int64_t lbm_hip_batch_steps_done(const lbm_batch *b) { return b ? b->steps : -1;
 
 size_t lbm_hip_batch_footprint(const lbm_batch *b) {
     if (!b) return 0;
-    return 2 * (size_t)b->pool_elems * esize(b->storage) +
+    const int bufs = 2 + (b->mid[0] != nullptr) + (b->mid[1] != nullptr);
+    return bufs * (size_t)b->pool_elems * esize(b->storage) +
            (size_t)(b->nitems + b->ntiles) * sizeof(int) + b->ents.size() * sizeof(BatchEntry);
 }
 

@@ -69,12 +69,13 @@ __device__ __forceinline__ void edge_task(const WaveGrid &g, int e, int &band, i
 // GL lanes, group k on chunk ck + k of a strip of at most GL - 2 rows (rows
 // Y0 .. Y0+nrows-1 on lanes 1 .. nrows of the group, halo rows on lanes 0 and
 // nrows+1: a group's owned rows read shifted values from their own group
-// only); GL = 64: one chunk, rows Y0 .. Y0+nrows-1 on lanes 1 .. nrows.  dstep: index of this launch's first collision relative to
+// only); GL = 64: the columns [xa, xb), rows Y0 .. Y0+nrows-1 on lanes
+// 1 .. nrows.  dstep: index of this launch's first collision relative to
 // the counter.
 template <typename S, int GL>
 __device__ __forceinline__ void edge_march(const Problem &p, const WaveGrid &g, uint32_t dstep,
                                            const S *__restrict__ P, S *__restrict__ Q, int Y0,
-                                           int nrows, int ck) {
+                                           int nrows, int ck, int xa = 0, int xb = 0) {
     using T = typename Calc<S>::T;
     constexpr bool THIN = GL < 64;
     const Layout &L = p.lay;
@@ -86,9 +87,9 @@ __device__ __forceinline__ void edge_march(const Problem &p, const WaveGrid &g, 
     const int lane = threadIdx.x;
     const int lr = lane & (GL - 1);                      // row Y0 - 1 + lr
     const int ckl = ck + lane / GL;
-    const bool active = ckl < g.ncx;                     // THIN: a strip's last wave may run short
+    const bool active = !THIN || ckl < g.ncx;            // THIN: a strip's last wave may run short
     const WaveSpan w = wave_span(L, g, 0, active ? ckl : g.ncx - 1);
-    const int X0 = w.X0, X1 = w.X1;
+    const int X0 = THIN ? w.X0 : xa, X1 = THIN ? w.X1 : xb;
     const int y = Y0 - 1 + lr;
     const bool row_in = active && y >= 1 && y <= L.ly;
     const bool owner = active && lr >= 1 && lr <= nrows && y <= L.ly;
@@ -194,13 +195,15 @@ __device__ __forceinline__ void wpanel_edge_wave(const Problem &p, const WaveGri
         const int ns = g.c_lo + (g.ncx - g.c_hi);
         const int i = e / ns, j = e % ns;
         const int Y0 = g.r0 + i * WAVE_ROWS;
+        const WaveSpan w = wave_span(p.lay, g, 0, j < g.c_lo ? j : g.c_hi + (j - g.c_lo));
         edge_march<S, 64>(p, g, dstep, P, Q, Y0, g.r1 - Y0 < WAVE_ROWS ? g.r1 - Y0 : WAVE_ROWS,
-                          j < g.c_lo ? j : g.c_hi + (j - g.c_lo));
+                          0, w.X0, w.X1);
         return;
     }
     int band, ck;
     edge_task(g, e, band, ck);
-    edge_march<S, 64>(p, g, dstep, P, Q, 1 + band * WAVE_ROWS, WAVE_ROWS, ck);
+    const WaveSpan w = wave_span(p.lay, g, 0, ck);
+    edge_march<S, 64>(p, g, dstep, P, Q, 1 + band * WAVE_ROWS, WAVE_ROWS, 0, w.X0, w.X1);
 }
 
 template <typename S, int GL>
@@ -320,7 +323,7 @@ __device__ __forceinline__ void wpanel_in_march(const Problem &p, const WaveGrid
     if (t >= g.nin) return;                              // whole waves only
     const int ck = g.c_lo + t / g.nbl;
     const int Y0 = g.r0 + (t % g.nbl) * ROWS;
-    const int X1 = L.lx + 1 - (g.ncx - 1 - ck) * g.cx;  // interior chunks are full
+    const int X1 = in_chunk_x1(g, ck);                  // interior chunks are full
     const int X0 = X1 - g.cx;
     const int y = Y0 - H + lane;
     // a partial last band: lanes past the interior's halo read row ly (in
@@ -451,7 +454,7 @@ __device__ __forceinline__ void wpanel_in_wave(const Problem &p, const WaveGrid 
     if (t >= g.nin) return;
     const int ck = g.c_lo + t / g.nbl;
     const int Y0 = g.r0 + (t % g.nbl) * (64 - 2 * H);
-    const int X1 = p.lay.lx + 1 - (g.ncx - 1 - ck) * g.cx;
+    const int X1 = in_chunk_x1(g, ck);
     const int X0 = X1 - g.cx;
     // FP16 keeps the branchy form everywhere: its split marches measured
     // slower (two-step -2%, four-step -16%, six-step -9%)
@@ -510,6 +513,8 @@ static WaveGrid wave_grid_max(const Layout &L, int cx, int cmax, int64_t waves) 
     g.r1 = 1 + b_hi * WAVE_ROWS;
     g.nbl = b_hi - b_lo;
     g.thin = g.nthin = g.gl = 0;
+    g.xr = L.lx + 1;
+    g.xl = 1;
     return g;
 }
 
@@ -660,69 +665,6 @@ hipError_t launch_wpanel(int storage, const Problem &p, const WaveGrid &g, const
     return e;
 }
 
-// ---- many channels, two steps per launch -----------------------------------------
-// The batched form of the panel sweep: every channel of a batch has its own
-// two-step wave grid (ChannelWaves); one launch of k_batch_wedge runs every
-// channel's perimeter waves, one of k_batch_win every channel's interior
-// waves, from flat work tables.  An interior workgroup's waves belong to one
-// channel (their lockstep barriers need the same march), and the workgroups
-// of a channel are contiguous in the table, so xcd_block keeps a channel's
-// neighbouring bands in one L2.  Each wave runs the single-lattice panel
-// sweep's own per-wave code, so a channel evolves bit for bit as it would
-// alone.
-
-template <typename S>
-__global__ __launch_bounds__(64) void k_batch_wedge(const ChannelWaves *__restrict__ ch,
-                                                    const int2 *__restrict__ items,
-                                                    const S *__restrict__ A, S *__restrict__ B) {
-    const int2 it = items[blockIdx.x];
-    const ChannelWaves c = ch[__builtin_amdgcn_readfirstlane(it.x)];
-    wpanel_edge_wave<S, 4>(c.p, c.g, __builtin_amdgcn_readfirstlane(it.y), 0u, A + c.off,
-                           B + c.off);
-}
-
-template <typename S>
-__global__ __launch_bounds__((64 * wpg_of<S, 2, 2>())) void k_batch_win(
-    const ChannelWaves *__restrict__ ch, const int2 *__restrict__ items, const S *__restrict__ A,
-    S *__restrict__ B) {
-    constexpr int WPG = wpg_of<S, 2, 2>();
-    const int2 it = items[xcd_block(blockIdx.x, gridDim.x)];
-    const ChannelWaves c = ch[__builtin_amdgcn_readfirstlane(it.x)];
-    const int t = __builtin_amdgcn_readfirstlane(it.y * WPG + (int)(threadIdx.x >> 6));
-    wpanel_in_wave<S, 2, 2>(c.p, c.g, t, 0u, A + c.off, B + c.off);
-}
-
-int batch_wpanel_wpg(int storage) {
-    return storage == 0 ? wpg_of<double, 2, 2>() : storage == 1 ? wpg_of<float, 2, 2>()
-                                                                : wpg_of<_Float16, 2, 2>();
-}
-
-// perimeter waves on `side`, interior waves on `st`, joined back into `st`
-// (they write disjoint nodes), as the single-lattice panel launch
-hipError_t launch_batch_wpanel(int storage, const ChannelWaves *ch, const int2 *edge_items,
-                               int nedge, const int2 *in_items, int nin_wgs, const void *A,
-                               void *B, hipStream_t st, hipStream_t side, hipEvent_t fork,
-                               hipEvent_t join) {
-    hipError_t e = hipEventRecord(fork, st);
-    if (e == hipSuccess) e = hipStreamWaitEvent(side, fork, 0);
-    if (e != hipSuccess) return e;
-    auto go = [&](auto tag) {
-        using S = decltype(tag);
-        if (nedge > 0)
-            k_batch_wedge<S><<<nedge, 64, 0, side>>>(ch, edge_items, (const S *)A, (S *)B);
-        if (nin_wgs > 0)
-            k_batch_win<S><<<nin_wgs, 64 * wpg_of<S, 2, 2>(), 0, st>>>(ch, in_items,
-                                                                      (const S *)A, (S *)B);
-    };
-    if (storage == 0) go(double{});
-    else if (storage == 1) go(float{});
-    else go(_Float16{});
-    e = hipGetLastError();
-    if (e == hipSuccess) e = hipEventRecord(join, side);
-    if (e == hipSuccess) e = hipStreamWaitEvent(st, join, 0);
-    return e;
-}
-
 // ---- rings of a few nodes inside the deep interior ---------------------------
 // The perimeter passes of a deep launch run on the frame around the deep
 // interior rectangle; each pass but the last must also leave its output on a
@@ -733,18 +675,12 @@ hipError_t launch_batch_wpanel(int storage, const ChannelWaves *ch, const int2 *
 // one thread per ring node recomputes the first step of its 3 x 3
 // neighbourhood from the pass's input and then the second step of the node,
 // the same operations as every other kernel, so the same bits.
-struct Ring {
-    int xa, xb, ya, yb;   // the deep interior rectangle [xa, xb) x [ya, yb)
-    int t;                // ring thickness in nodes
-    int n;                // nodes in the ring
-};
-
+// Ring node k of r.
 template <typename S>
-__global__ __launch_bounds__(256) void k_ring2(Problem p, Ring r, uint32_t dstep,
-                                               const S *__restrict__ P, S *__restrict__ Q) {
+__device__ __forceinline__ void ring2_node(const Problem &p, const Ring &r, int k, uint32_t dstep,
+                                           const S *__restrict__ P, S *__restrict__ Q) {
     using T = typename Calc<S>::T;
     const Layout &L = p.lay;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= r.n) return;
     // y runs fastest (neighbouring threads read neighbouring elements): the
     // bottom and top rows column by column, then the left and right columns
@@ -780,6 +716,24 @@ __global__ __launch_bounds__(256) void k_ring2(Problem p, Ring r, uint32_t dstep
     for_pops([&]<int I>() { Q[I * L.plane + n] = to_storage<S>(h[I]); });
 }
 
+template <typename S>
+__global__ __launch_bounds__(256) void k_ring2(Problem p, Ring r, uint32_t dstep,
+                                               const S *__restrict__ P, S *__restrict__ Q) {
+    ring2_node<S>(p, r, (int)(blockIdx.x * blockDim.x + threadIdx.x), dstep, P, Q);
+}
+
+// the deep interior rectangle of grid g
+static Ring deep_rect(const Layout &L, const WaveGrid &g) {
+    Ring r;
+    r.xa = L.lx + 1 - (g.ncx - g.c_lo) * g.cx;
+    r.xb = L.lx + 1 - (g.ncx - g.c_hi) * g.cx;
+    r.ya = g.r0;
+    r.yb = g.r1;
+    r.t = r.n = 0;
+    return r;
+}
+
+
 hipError_t launch_wpanel_deep(int storage, const Problem &p, int cx, int levels, const void *P,
                               void *const *mid, void *Q, hipStream_t st, hipStream_t side,
                               hipEvent_t fork, hipEvent_t join, KernelProbe *probe) {
@@ -791,11 +745,7 @@ hipError_t launch_wpanel_deep(int storage, const Problem &p, int cx, int levels,
     // the eight-step FP64 march (two columns in flight need 265 registers,
     // one wave per SIMD; one column 252, two waves)
     const int depth = g.cx % 2 == 0 && !(levels == 8 && storage == 0) ? 2 : 1;
-    Ring ring;
-    ring.xa = L.lx + 1 - (g.ncx - g.c_lo) * g.cx;
-    ring.xb = L.lx + 1 - (g.ncx - g.c_hi) * g.cx;
-    ring.ya = g.r0;
-    ring.yb = g.r1;
+    const Ring ring = deep_rect(L, g);
     hipError_t e = hipEventRecord(fork, st);
     if (e == hipSuccess) e = hipStreamWaitEvent(side, fork, 0);
     if (e != hipSuccess) return e;
@@ -803,21 +753,14 @@ hipError_t launch_wpanel_deep(int storage, const Problem &p, int cx, int levels,
         using T = decltype(tag);
         // perimeter pass j advances the frame by steps 2j+1, 2j+2, from P or
         // mid[j-1] into mid[j] or Q; all but the last also fill a ring inside
-        // the interior rectangle: 2 nodes deep for the pass before the last
-        // (the perimeter waves' halo computes a first step one node in, from
-        // pulls two nodes in), and 2 deeper for each pass before that (a
-        // ring node t-1 deep takes its neighbour's first step t deep, from
-        // pulls t+1 deep)
+        // the interior rectangle (ring_depth)
         for (int j = 0; j <= nmid; ++j) {
             const T *src = (const T *)(j == 0 ? P : mid[j - 1]);
             T *dst = (T *)(j == nmid ? Q : mid[j]);
             if (g.gl == 16) k_wpanel_edge<T, 16><<<g.nedge, 64, 0, side>>>(p, g, 2u * j, src, dst);
             else k_wpanel_edge<T, 8><<<g.nedge, 64, 0, side>>>(p, g, 2u * j, src, dst);
             if (j < nmid) {
-                Ring r = ring;
-                r.t = 2 + 2 * (nmid - 1 - j);
-                const int wx = r.xb - r.xa, wy = r.yb - r.ya;
-                r.n = 2 * r.t * wx + 2 * r.t * (wy - 2 * r.t);
+                const Ring r = ring_of(ring, ring_depth(nmid, j));
                 k_ring2<T><<<(r.n + 255) / 256, 256, 0, side>>>(p, r, 2u * j, src, dst);
             }
         }
@@ -833,6 +776,230 @@ hipError_t launch_wpanel_deep(int storage, const Problem &p, int cx, int levels,
             if (depth == 2) launch_in<T, 2, 8>(p, g, 0u, src, dst, st, probe);
             else launch_in<T, 1, 8>(p, g, 0u, src, dst, st, probe);
         }
+    };
+    if (storage == 0) go(double{});
+    else if (storage == 1) go(float{});
+    else go(_Float16{});
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventRecord(join, side);
+    if (e == hipSuccess) e = hipStreamWaitEvent(st, join, 0);
+    return e;
+}
+
+// ---- many channels per launch ----------------------------------------------------
+// The batched forms of the register marches: every channel of a batch has its
+// own wave grid (ChannelWaves); one launch runs a pass of every channel's
+// perimeter waves, one every channel's interior waves, one every channel's
+// ring, from flat work tables.  An interior workgroup's waves belong to one
+// channel (their lockstep barriers need the same march), and the workgroups
+// of a channel are contiguous in the table, so xcd_block keeps a channel's
+// neighbouring bands in one L2.  Each wave and ring thread runs the
+// single-lattice sweep's own code, so a channel evolves bit for bit as it
+// would alone.
+
+template <typename S, int GL>
+__global__ __launch_bounds__(64) void k_batch_wedge(const ChannelWaves *__restrict__ ch,
+                                                    const int2 *__restrict__ items, uint32_t dstep,
+                                                    const S *__restrict__ A, S *__restrict__ B) {
+    const int2 it = items[blockIdx.x];
+    const ChannelWaves c = ch[__builtin_amdgcn_readfirstlane(it.x)];
+    wpanel_edge_wave<S, GL>(c.p, c.g, __builtin_amdgcn_readfirstlane(it.y), dstep, A + c.off,
+                            B + c.off);
+}
+
+template <typename S, int DEPTH, int LEVELS>
+__global__ __launch_bounds__((64 * wpg_of<S, DEPTH, LEVELS>())) void k_batch_win(
+    const ChannelWaves *__restrict__ ch, const int2 *__restrict__ items, uint32_t dstep,
+    const S *__restrict__ A, S *__restrict__ B) {
+    constexpr int WPG = wpg_of<S, DEPTH, LEVELS>();
+    const int2 it = items[xcd_block(blockIdx.x, gridDim.x)];
+    const ChannelWaves c = ch[__builtin_amdgcn_readfirstlane(it.x)];
+    const int t = __builtin_amdgcn_readfirstlane(it.y * WPG + (int)(threadIdx.x >> 6));
+    wpanel_in_wave<S, DEPTH, LEVELS>(c.p, c.g, t, dstep, A + c.off, B + c.off);
+}
+
+// Perimeter wave e of a pass of a batched deep march's frame (DeepFrame)
+// with sr strip rows, nsw waves per strip and nsb side bands: the bottom
+// strip's waves, the top strip's, then the left side's bands and the right
+// side's.  GL: lanes per strip chunk (sr + 2 <= GL).
+template <typename S, int GL>
+__device__ __forceinline__ void frame_wave(const Problem &p, const WaveGrid &sg, int sr, int nsw,
+                                           int nsb, int e, uint32_t dstep,
+                                           const S *__restrict__ P, S *__restrict__ Q) {
+    const Layout &L = p.lay;
+    if (e < 2 * nsw) {
+        const bool top = e >= nsw;
+        const int ck0 = (top ? e - nsw : e) * (64 / GL);
+        edge_march<S, GL>(p, sg, dstep, P, Q, top ? L.ly + 1 - sr : 1, sr, ck0);
+        return;
+    }
+    e -= 2 * nsw;
+    const bool right = e >= nsb;
+    const int Y0 = 1 + sr + (right ? e - nsb : e) * WAVE_ROWS;
+    const int Y1 = L.ly + 1 - sr;
+    edge_march<S, 64>(p, sg, dstep, P, Q, Y0, Y1 - Y0 < WAVE_ROWS ? Y1 - Y0 : WAVE_ROWS, 0,
+                      right ? L.lx + 1 - sr : 1, right ? L.lx + 1 : 1 + sr);
+}
+
+// Pass j's per-channel counts are read from the table in global memory: a
+// private copy of the record indexed by the run-time j would live in
+// scratch memory.
+template <typename S, int GL>
+__global__ __launch_bounds__(64) void k_batch_frame(const ChannelWaves *__restrict__ ch,
+                                                    const int2 *__restrict__ items, int j,
+                                                    uint32_t dstep, const S *__restrict__ A,
+                                                    S *__restrict__ B) {
+    const int2 it = items[blockIdx.x];
+    const int ci = __builtin_amdgcn_readfirstlane(it.x);
+    const ChannelWaves *cw = ch + ci;
+    const int sr = cw->f.sr[j], nsw = cw->f.nsw[j], nsb = cw->f.nsb[j];
+    const Problem p = cw->p;
+    const WaveGrid sg = cw->f.sg;
+    const int64_t off = cw->off;
+    frame_wave<S, GL>(p, sg, sr, nsw, nsb, __builtin_amdgcn_readfirstlane(it.y), dstep, A + off,
+                      B + off);
+}
+
+int batch_wpanel_wpg(int storage) {
+    return storage == 0 ? wpg_of<double, 2, 2>() : storage == 1 ? wpg_of<float, 2, 2>()
+                                                                : wpg_of<_Float16, 2, 2>();
+}
+
+// the batched deep marches run two columns of loads in flight (even chunk
+// widths only)
+int batch_deep_wpg(int storage, int levels) {
+    if (levels == 4)
+        return storage == 0 ? wpg_of<double, 2, 4>() : storage == 1 ? wpg_of<float, 2, 4>()
+                                                                    : wpg_of<_Float16, 2, 4>();
+    return storage == 0 ? wpg_of<double, 2, 6>() : storage == 1 ? wpg_of<float, 2, 6>()
+                                                                : wpg_of<_Float16, 2, 6>();
+}
+
+// lanes per strip chunk of a frame pass with sr strip rows (and two halo rows)
+constexpr int frame_gl(int sr) { return sr + 2 <= 8 ? 8 : 16; }
+
+// strip rows of pass j of a LEVELS-step frame (0 past the last pass)
+constexpr int frame_rows(int levels, int j) {
+    const int nmid = levels / 2 - 1;
+    return j > nmid ? 0 : levels + (j < nmid ? ring_depth(nmid, j) : 0);
+}
+
+// The interior grid and frame of a batched deep march (DeepFrame): interior
+// rows [H + 2, ly - H) in (64 - 2H)-row bands, columns [H + 2, lx - H) in
+// cx-column chunks right-aligned at lx - H (the first one moved right to
+// start at H + 2); false when the channel has less than one chunk of
+// interior columns or one interior row, or its widest strips or sides meet.
+bool batch_deep_grid(const Layout &L, int cx, int levels, WaveGrid *g, DeepFrame *f) {
+    if ((levels != 4 && levels != 6) || cx < 4 || cx % 2 || !wave_grid_fits(L)) return false;
+    const int H = levels - 1, nmid = levels / 2 - 1;
+    const int xl = H + 2, xr = L.lx - H, r0 = H + 2, r1 = L.ly - H;
+    const int sr0 = frame_rows(levels, 0);
+    if (xr - xl < cx || r1 - r0 < 1 || 2 * sr0 > L.ly || 2 * sr0 > L.lx) return false;
+    // 32-bit offsets of the interior kernel's buffer resources
+    if ((int64_t)(cx + 16) * L.pitch * 8 >= (int64_t)DROP) return false;
+    WaveGrid w = wave_grid_max(L, cx, cx, 1);   // cx, ncx, the strip chunks
+    f->sg = w;
+    f->nmid = nmid;
+    for (int j = 0; j < 3; ++j) {
+        const int sr = frame_rows(levels, j);
+        f->sr[j] = sr;
+        f->nsw[j] = j <= nmid ? (w.ncx + 64 / frame_gl(sr) - 1) / (64 / frame_gl(sr)) : 0;
+        f->nsb[j] = j <= nmid ? (L.ly - 2 * sr + WAVE_ROWS - 1) / WAVE_ROWS : 0;
+    }
+    w.xl = xl;
+    w.xr = xr;
+    w.ncx = (xr - xl + cx - 1) / cx;
+    w.c_lo = 0;
+    w.c_hi = w.ncx;
+    w.r0 = r0;
+    w.r1 = r1;
+    w.nbl = (r1 - r0 + 64 - 2 * H - 1) / (64 - 2 * H);
+    w.nin = w.nbl * w.ncx;
+    w.b_lo = w.b_hi = 0;
+    w.thin = 1;
+    w.nthin = w.nedge = 0;
+    w.gl = 0;
+    *g = w;
+    return true;
+}
+
+// perimeter waves of pass j
+int batch_frame_waves(const DeepFrame &f, int j) { return 2 * f.nsw[j] + 2 * f.nsb[j]; }
+
+// one two-step pass of the tables' channels (perimeter then interior waves,
+// both on `q`: they write disjoint nodes)
+template <typename S>
+static void batch_pair_on(const BatchTables &t, uint32_t dstep, const S *A, S *B, hipStream_t q) {
+    if (t.nedge > 0) k_batch_wedge<S, 4><<<t.nedge, 64, 0, q>>>(t.ch, t.edge, dstep, A, B);
+    if (t.nin_wgs > 0)
+        k_batch_win<S, 2, 2><<<t.nin_wgs, 64 * wpg_of<S, 2, 2>(), 0, q>>>(t.ch, t.in, dstep, A, B);
+}
+
+// perimeter waves on `side`, interior waves on `st`, joined back into `st`
+// (they write disjoint nodes), as the single-lattice panel launch
+hipError_t launch_batch_wpanel(int storage, const BatchTables &t, uint32_t dstep, const void *A,
+                               void *B, hipStream_t st, hipStream_t side, hipEvent_t fork,
+                               hipEvent_t join) {
+    hipError_t e = hipEventRecord(fork, st);
+    if (e == hipSuccess) e = hipStreamWaitEvent(side, fork, 0);
+    if (e != hipSuccess) return e;
+    auto go = [&](auto tag) {
+        using S = decltype(tag);
+        if (t.nedge > 0)
+            k_batch_wedge<S, 4><<<t.nedge, 64, 0, side>>>(t.ch, t.edge, dstep, (const S *)A, (S *)B);
+        if (t.nin_wgs > 0)
+            k_batch_win<S, 2, 2><<<t.nin_wgs, 64 * wpg_of<S, 2, 2>(), 0, st>>>(
+                t.ch, t.in, dstep, (const S *)A, (S *)B);
+    };
+    if (storage == 0) go(double{});
+    else if (storage == 1) go(float{});
+    else go(_Float16{});
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventRecord(join, side);
+    if (e == hipSuccess) e = hipStreamWaitEvent(st, join, 0);
+    return e;
+}
+
+// The deep channels' interior waves march LEVELS steps on `st` (A -> B);
+// on `side`, their frame advances two steps per pass (A -> mid[0] -> ... ->
+// B, each pass but the last widened by what the next one reads), then the
+// shallow channels take LEVELS/2 two-step passes through the same buffers.
+hipError_t launch_batch_wpanel_deep(int storage, int levels, const BatchTables &deep,
+                                    const int2 *const *frame_items, const int *nframe,
+                                    const BatchTables &shallow, const void *A, void *const *mid,
+                                    void *B, hipStream_t st, hipStream_t side, hipEvent_t fork,
+                                    hipEvent_t join) {
+    if (levels != 4 && levels != 6) return hipErrorInvalidValue;
+    const int nmid = levels / 2 - 1;
+    hipError_t e = hipEventRecord(fork, st);
+    if (e == hipSuccess) e = hipStreamWaitEvent(side, fork, 0);
+    if (e != hipSuccess) return e;
+    auto go = [&](auto tag) {
+        using S = decltype(tag);
+        auto src_of = [&](int j) { return (const S *)(j == 0 ? A : mid[j - 1]); };
+        auto dst_of = [&](int j) { return (S *)(j == nmid ? B : mid[j]); };
+        // the interior waves are enqueued first: queued behind the frame's
+        // perimeter waves they waited for slots while those ran at low
+        // occupancy (512 channels, FP32 four steps: 91.7 -> 98.5 GLUPS,
+        // FP64 six steps 85.3 -> 87.5, FP16 four steps 127.2 -> 134.8)
+        if (deep.nin_wgs > 0) {
+            if (levels == 4)
+                k_batch_win<S, 2, 4><<<deep.nin_wgs, 64 * wpg_of<S, 2, 4>(), 0, st>>>(
+                    deep.ch, deep.in, 0u, (const S *)A, (S *)B);
+            else
+                k_batch_win<S, 2, 6><<<deep.nin_wgs, 64 * wpg_of<S, 2, 6>(), 0, st>>>(
+                    deep.ch, deep.in, 0u, (const S *)A, (S *)B);
+        }
+        for (int j = 0; j <= nmid; ++j) {
+            if (nframe[j] <= 0) continue;
+            if (frame_gl(frame_rows(levels, j)) == 8)
+                k_batch_frame<S, 8><<<nframe[j], 64, 0, side>>>(deep.ch, frame_items[j], j, 2u * j,
+                                                               src_of(j), dst_of(j));
+            else
+                k_batch_frame<S, 16><<<nframe[j], 64, 0, side>>>(deep.ch, frame_items[j], j,
+                                                                2u * j, src_of(j), dst_of(j));
+        }
+        for (int j = 0; j <= nmid; ++j) batch_pair_on<S>(shallow, 2u * j, src_of(j), dst_of(j), side);
     };
     if (storage == 0) go(double{});
     else if (storage == 1) go(float{});

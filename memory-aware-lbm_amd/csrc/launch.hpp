@@ -85,7 +85,18 @@ struct WaveGrid {
     // perimeter wave: nthin waves) and the edge chunks of rows [r0, r1) in
     // 62-row bands
     int thin, nthin, gl;
+    // interior chunks are right-aligned at column xr (exclusive end of the
+    // last chunk; lx + 1 for the single-lattice grids); a chunk that would
+    // start left of xl is moved right to start at xl (it then overlaps its
+    // neighbour, which writes the same values)
+    int xr, xl;
 };
+
+// right end (exclusive) of interior chunk ck of g
+__host__ __device__ inline int in_chunk_x1(const WaveGrid &g, int ck) {
+    const int X1 = g.xr - (g.ncx - 1 - ck) * g.cx;
+    return X1 - g.cx < g.xl ? g.xl + g.cx : X1;
+}
 
 // Rows [Y0-1, Y0+62] (lanes) and columns [X0, X1) owned by wave (band, ck);
 // chunks are aligned to the right end of the channel.
@@ -116,21 +127,86 @@ hipError_t launch_wpanel_deep(int storage, const Problem &p, int cx, int levels,
                               void *const *mid, void *Q, hipStream_t st, hipStream_t side,
                               hipEvent_t fork, hipEvent_t join, KernelProbe *probe = nullptr);
 
-// Many channels of a batch (pool offset `off` elements), two steps per
-// launch with the panel sweep's register march over each channel's wave
-// grid: perimeter waves (edge_items: channel, wave) on `side`, interior
-// waves (in_items: channel, first wave / batch_wpanel_wpg of every
-// workgroup, the workgroups of a channel contiguous) on `st`, joined back
-// into `st`.  The channels' P and Q sit at A + off and B + off.
+// The deep interior rectangle [xa, xb) x [ya, yb) of a deep grid and a ring
+// of t nodes just inside it (n nodes), which a deep launch's perimeter passes
+// but the last also fill.
+struct Ring {
+    int xa, xb, ya, yb;
+    int t;
+    int n;
+};
+
+// the ring of rectangle r, t nodes thick
+__host__ __device__ inline Ring ring_of(Ring r, int t) {
+    r.t = t;
+    const int wx = r.xb - r.xa, wy = r.yb - r.ya;
+    r.n = 2 * t * wx + 2 * t * (wy - 2 * t);
+    return r;
+}
+
+// thickness of the ring that perimeter pass j (of nmid + 1) fills: 2 nodes
+// for the pass before the last (the perimeter waves' halo computes a first
+// step one node in, from pulls two nodes in), 2 deeper for each pass before
+// that (a ring node t-1 deep takes its neighbour's first step t deep, from
+// pulls t+1 deep)
+__host__ __device__ constexpr int ring_depth(int nmid, int j) { return 2 + 2 * (nmid - 1 - j); }
+
+// Many channels of a batch (pool offset `off` elements), several steps per
+// launch with the register marches over each channel's wave grid.  The
+// channels' P and Q sit at A + off and B + off.  A work table lists
+// (channel, wave) for perimeter waves, (channel, first wave / WPG) for the
+// workgroups of interior waves (the workgroups of a channel contiguous) and
+// (channel, first node / 256) for ring nodes.
+// The frame of a batched deep march (LEVELS = 2 (nmid + 1) steps, H = LEVELS
+// - 1): its interior waves cover rows [H + 2, ly - H) and columns
+// [H + 2, lx - H); perimeter pass j (steps 2j + 1, 2j + 2) covers the
+// frame around them widened inwards by the depth t_j its successor reads
+// (t_nmid = 0, t_j = t_{j+1} + 2: the next pass's halo computes a first step
+// one node further in, from pulls two nodes in), so no ring kernel is needed:
+// strips of rows [1, 1 + sr_j) and [ly + 1 - sr_j, ly + 1) over the whole
+// width (chunks of the strip grid sg, 64 / gl per wave, gl = 8 lanes per
+// chunk for sr_j + 2 <= 8, else 16), and the columns
+// [1, 1 + sr_j) and [lx + 1 - sr_j, lx + 1) of the rows between them in
+// 62-row bands, with sr_j = H + 1 + t_j.
+struct DeepFrame {
+    int nmid;            // passes - 1
+    int sr[3];           // strip rows (= side columns) of each pass
+    int nsw[3];          // waves per strip of each pass
+    int nsb[3];          // side bands of each pass, per side
+    WaveGrid sg;         // the strips' chunks (cx, ncx; right-aligned at lx + 1)
+};
+
 struct ChannelWaves {
     Problem p;
     WaveGrid g;
+    DeepFrame f;     // deep grids only
     int64_t off;
 };
+struct BatchTables {
+    const ChannelWaves *ch;
+    const int2 *edge;
+    int nedge;
+    const int2 *in;
+    int nin_wgs;
+};
+// Two steps: perimeter waves on `side`, interior waves on `st`, joined back
+// into `st`; dstep: index of the first collision relative to the counter.
 int batch_wpanel_wpg(int storage);
-hipError_t launch_batch_wpanel(int storage, const ChannelWaves *ch, const int2 *edge_items,
-                               int nedge, const int2 *in_items, int nin_wgs, const void *A,
+hipError_t launch_batch_wpanel(int storage, const BatchTables &t, uint32_t dstep, const void *A,
                                void *B, hipStream_t st, hipStream_t side, hipEvent_t fork,
                                hipEvent_t join);
+// LEVELS = 4 or 6 steps: the channels with a deep grid (deep: its interior
+// waves on `st`; its perimeter passes, frame_items[j] (channel, wave), on
+// `side` through mid[0 .. LEVELS/2 - 2]) and the others (shallow, on their
+// two-step grids: LEVELS/2 two-step launches on `side` through the same
+// buffers).  batch_deep_grid: false when the channel is too small for it.
+bool batch_deep_grid(const Layout &L, int cx, int levels, WaveGrid *g, DeepFrame *f);
+int batch_frame_waves(const DeepFrame &f, int j);
+int batch_deep_wpg(int storage, int levels);
+hipError_t launch_batch_wpanel_deep(int storage, int levels, const BatchTables &deep,
+                                    const int2 *const *frame_items, const int *nframe,
+                                    const BatchTables &shallow, const void *A, void *const *mid,
+                                    void *B, hipStream_t st, hipStream_t side, hipEvent_t fork,
+                                    hipEvent_t join);
 
 }  // namespace lbm

@@ -157,15 +157,25 @@ class Batch:
             self.launch["tile_cols"] = tile_cols
 
     def set_sweep(self, variant: str, chunk_cols: int = 0):
-        """The two-step launches' sweep: "tight_block" (LDS tiles) or
-        "panel" (the register march over every channel's wave grid, chunks
-        of chunk_cols columns; 0 keeps the width)."""
+        """The multi-step launches' sweep: "tight_block" (two steps, LDS
+        tiles), "panel" (two steps, the register march over every channel's
+        wave grid, chunks of chunk_cols columns), "panel4" / "panel6" (four /
+        six steps, the deep register marches over the channels with a deep
+        interior, chunk_cols-column chunks, two-step passes of the panel
+        march for the others; remainders of two steps take the panel march).
+        chunk_cols 0 keeps the width."""
         from . import VARIANT
         _check(library().lbm_hip_batch_set_sweep(self._h, VARIANT[variant], int(chunk_cols)),
                "batch_set_sweep")
         self.launch["sweep"] = variant
         if chunk_cols:
             self.launch["chunk_cols"] = chunk_cols
+
+    def steps_per_launch(self) -> int:
+        """Steps of a full multi-step launch with the current settings."""
+        if self.launch["depth"] < 2:
+            return 1
+        return {"panel4": 4, "panel6": 6}.get(self.launch["sweep"], 2)
 
     def step(self, n: int):
         _check(library().lbm_hip_batch_step(self._h, int(n)), "batch_step")
@@ -208,8 +218,9 @@ def batch_key(batch: Batch) -> str:
 
 def autotune(batch: Batch, steps: int = 20, graph_steps: int = 10,
              ty_caps=(32, 64, 128, 256), wgs=(2, 4, 8, 16), tile_cols=None,
-             panel_cols=(4, 6, 8), cache_dir: Optional[str] = TUNE_DIR,
-             use_cache: bool = True, repeats: int = 2) -> dict:
+             panel_cols=(4, 6, 8), deep_cols=(12, 16, 24, 32, 48), deep_graph_steps: int = 12,
+             cache_dir: Optional[str] = TUNE_DIR, use_cache: bool = True,
+             repeats: int = 2) -> dict:
     """Pick the launch shape with the highest throughput on this batch.
 
     Every candidate runs `steps` steps `repeats` times (after one warm-up
@@ -217,26 +228,31 @@ def autotune(batch: Batch, steps: int = 20, graph_steps: int = 10,
     advancing, which does not change the per-step cost.  Candidates: the one-step launch over ty_caps x wgs and
     the two-step launch over tile_cols (default: widths around three and two
     workgroups per CU for the storage format; skipped when a channel is
-    narrower than four columns), and the two-step register march over
-    panel_cols-column chunks.  Returns {"depth", "tile_cols", "ty_cap",
-    "wgs_per_cu", "graph_steps", "sweep", "chunk_cols", "glups",
-    "candidates": [...]}; cached by batch_key."""
+    narrower than four columns), the two-step register march over
+    panel_cols-column chunks, and the four- and six-step register marches
+    over deep_cols-column chunks (hipGraphs of deep_graph_steps steps, a
+    multiple of both, timed over twice that).  Returns {"depth",
+    "tile_cols", "ty_cap", "wgs_per_cu", "graph_steps", "sweep",
+    "chunk_cols", "steps_per_launch", "glups", "candidates": [...]}; cached
+    by batch_key."""
     key = batch_key(batch)
     path = os.path.join(cache_dir, key + ".json") if cache_dir else None
     if use_cache and path and os.path.exists(path):
         best = json.load(open(path))
         batch.set_launch(best["wgs_per_cu"], best["ty_cap"], best["graph_steps"])
         batch.set_depth(best.get("depth", 1), best.get("tile_cols", 0))
+        if best.get("panel_cols"):
+            batch.set_sweep("panel", best["panel_cols"])
         batch.set_sweep(best.get("sweep", "tight_block"), best.get("chunk_cols", 0))
         best["cached"] = True
         return best
     cells = batch.cells
     results = []
 
-    def timed(**shape):
-        batch.time_steps(graph_steps)  # capture + warm
-        ms = min(batch.time_steps(steps) for _ in range(max(1, repeats)))
-        results.append(dict(shape, glups=cells * steps / (ms * 1e-3) / 1e9))
+    def timed(gs=graph_steps, n=steps, **shape):
+        batch.time_steps(gs)  # capture + warm
+        ms = min(batch.time_steps(n) for _ in range(max(1, repeats)))
+        results.append(dict(shape, graph_steps=gs, glups=cells * n / (ms * 1e-3) / 1e9))
 
     batch.set_depth(1)
     for ty in ty_caps:
@@ -252,21 +268,39 @@ def autotune(batch: Batch, steps: int = 20, graph_steps: int = 10,
             timed(depth=2, tile_cols=tc, ty_cap=batch.launch["ty_cap"],
                   wgs_per_cu=batch.launch["wgs_per_cu"], sweep="tight_block")
         try:
+            best_pc, best_g = panel_cols[0] if panel_cols else 0, -1.0
             for pc in panel_cols:
                 batch.set_sweep("panel", pc)
                 timed(depth=2, tile_cols=batch.launch["tile_cols"], ty_cap=batch.launch["ty_cap"],
                       wgs_per_cu=batch.launch["wgs_per_cu"], sweep="panel", chunk_cols=pc)
+                if results[-1]["glups"] > best_g:
+                    best_pc, best_g = pc, results[-1]["glups"]
+            if deep_cols:
+                # the deep sweeps' two-step passes take the best two-step width
+                if best_pc:
+                    batch.set_sweep("panel", best_pc)
+                batch.set_launch(0, 0, deep_graph_steps)
+                for sweep in ("panel4", "panel6"):
+                    for dc in deep_cols:
+                        batch.set_sweep(sweep, dc)
+                        timed(gs=deep_graph_steps, n=2 * deep_graph_steps, depth=2,
+                              tile_cols=batch.launch["tile_cols"], ty_cap=batch.launch["ty_cap"],
+                              wgs_per_cu=batch.launch["wgs_per_cu"], sweep=sweep, chunk_cols=dc,
+                              panel_cols=best_pc)
         except Exception:       # a channel the register march refuses
             pass
     for r in results:
         r.setdefault("sweep", "tight_block")
         r.setdefault("chunk_cols", 0)
     best = max(results, key=lambda r: r["glups"])
+    spl = 1 if best["depth"] < 2 else {"panel4": 4, "panel6": 6}.get(best["sweep"], 2)
     out = {"depth": best["depth"], "tile_cols": best["tile_cols"], "ty_cap": best["ty_cap"],
-           "wgs_per_cu": best["wgs_per_cu"], "graph_steps": graph_steps, "sweep": best["sweep"],
-           "chunk_cols": best["chunk_cols"], "glups": best["glups"], "key": key,
+           "wgs_per_cu": best["wgs_per_cu"], "graph_steps": best["graph_steps"],
+           "sweep": best["sweep"], "chunk_cols": best["chunk_cols"],
+           "panel_cols": best.get("panel_cols", 0), "steps_per_launch": spl,
+           "glups": best["glups"], "key": key,
            "tuned_at": time.strftime("%Y-%m-%d %H:%M:%S"), "candidates": results}
-    batch.set_launch(out["wgs_per_cu"], out["ty_cap"], graph_steps)
+    batch.set_launch(out["wgs_per_cu"], out["ty_cap"], out["graph_steps"])
     batch.set_depth(out["depth"], out["tile_cols"])
     batch.set_sweep(out["sweep"], out["chunk_cols"])
     if path:

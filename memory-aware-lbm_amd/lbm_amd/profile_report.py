@@ -8,7 +8,7 @@ Inputs are three rocprofv3 runs of the same command (CSV output):
 
 and, optionally, the JSON line bench.py printed in the traced run (with
 --batch-cells N: only the batch kernels of a batching trace of N cells and
-whole two-step launch times, from the trace alone).  The report
+whole launch times, from the trace alone).  The report
 gives, per kernel, the dispatch count and the average / minimum duration, the
 algorithmic bytes of one dispatch (bytes per lattice update -- 144 FP64, 72
 FP32, 36 FP16 -- times the nodes the dispatch updates once, i.e. one read and
@@ -186,24 +186,41 @@ def gap_report(trace_dir: str, label: str) -> List[str]:
     return out + [""]
 
 
-def batch_marches(trace_dir: str) -> Dict[str, List[float]]:
-    """Whole two-step launches of the batched register march, per storage
-    type: a perimeter dispatch (k_batch_wedge, second stream) and the next
-    interior dispatch (k_batch_win) run side by side, so one launch lasts
-    from the earlier start to the later end (ms)."""
+def march_levels(name: str) -> int:
+    """Steps of a k_batch_win<S, DEPTH, LEVELS> dispatch (2 without the third
+    template argument)."""
+    args = name[name.find("<") + 1:name.rfind(">")].split(",")
+    return int(args[2]) if len(args) >= 3 else 2
+
+
+def batch_marches(trace_dir: str) -> Dict[Tuple[str, int], List[float]]:
+    """Whole launches of the batched register marches, per (storage type,
+    steps): a two-step launch is a perimeter dispatch (k_batch_wedge, second
+    stream) and the next interior dispatch (k_batch_win<S, 2, 2>); a four- or
+    six-step launch is its frame passes (k_batch_frame) and the two-step
+    passes of its small channels on the second stream, then the interior
+    dispatch (k_batch_win<S, 2, 4 or 6>).  The dispatches of a launch run side
+    by side, so it lasts from the earliest start to the latest end (ms)."""
     rows = sorted((int(r["Dispatch_Id"]), short(r["Kernel_Name"]), int(r["Start_Timestamp"]),
                    int(r["End_Timestamp"])) for r in _rows(trace_dir, "kernel_trace.csv"))
-    out: Dict[str, List[float]] = defaultdict(list)
-    edge = None
+    out: Dict[Tuple[str, int], List[float]] = defaultdict(list)
+    group: List[Tuple[int, int]] = []
+    deep = False
     for _, name, t0, t1 in rows:
-        if name.startswith("k_batch_wedge"):
-            edge = (t0, t1)
+        if name.startswith("k_batch_frame"):
+            if not deep:
+                group, deep = [], True
+            group.append((t0, t1))
+        elif name.startswith("k_batch_wedge"):
+            group.append((t0, t1))
         elif name.startswith("k_batch_win"):
-            s, e = (min(t0, edge[0]), max(t1, edge[1])) if edge else (t0, t1)
-            out[stype(name)].append((e - s) * 1e-6)
-            edge = None
-        else:
-            edge = None
+            group.append((t0, t1))
+            lv = march_levels(name)
+            if lv == 2 and deep:
+                continue                      # a small channel's pass inside a deep launch
+            s, e = min(g[0] for g in group), max(g[1] for g in group)
+            out[(stype(name), lv)].append((e - s) * 1e-6)
+            group, deep = [], False
     return out
 
 
@@ -213,20 +230,21 @@ def batch_report(trace_dir: str, cells: int) -> List[str]:
            "| kernel | calls | avg ms | min ms |", "|---|---|---|---|"]
     for name, v in sorted(kt.items(), key=lambda kv: -sum(kv[1])):
         out.append(f"| {name} | {len(v)} | {sum(v) / len(v):.4f} | {min(v):.4f} |")
-    out += ["", "## Whole two-step launches", "",
-            f"Two updates of all {cells:,} cells per launch; algorithmic bytes per launch = "
-            "bytes per lattice update (FP64 144, FP32 72, FP16 36) x 2 x cells.  A register "
-            "march launch lasts from the first start to the last end of its perimeter and "
-            "interior dispatches; an LDS-tile launch is one k_batch_block dispatch.", "",
-            "| launch | launches | avg ms | min ms | algorithmic GB/launch | algorithmic GB/s "
-            "| GLUPS |", "|---|---|---|---|---|---|---|"]
-    rows = [(f"march<{t}>", v) for t, v in batch_marches(trace_dir).items()]
-    rows += [(n, v) for n, v in kt.items() if n.startswith("k_batch_block")]
-    for name, v in rows:
+    out += ["", "## Whole launches", "",
+            f"k updates of all {cells:,} cells per k-step launch; algorithmic bytes per launch = "
+            "bytes per lattice update (FP64 144, FP32 72, FP16 36) x cells (one read and one "
+            "write of every population).  A register march launch lasts from the first start to "
+            "the last end of its dispatches; an LDS-tile launch is one k_batch_block dispatch "
+            "(two steps).", "",
+            "| launch | steps | launches | avg ms | min ms | algorithmic GB/launch | "
+            "algorithmic GB/s | GLUPS |", "|---|---|---|---|---|---|---|---|"]
+    rows = [(f"march<{t}>", lv, v) for (t, lv), v in sorted(batch_marches(trace_dir).items())]
+    rows += [(n, 2, v) for n, v in kt.items() if n.startswith("k_batch_block")]
+    for name, lv, v in rows:
         avg = sum(v) / len(v)
-        gb = BPU[stype(name)] * 2 * cells / 1e9
-        out.append(f"| {name} | {len(v)} | {avg:.4f} | {min(v):.4f} | {gb:.3f} | "
-                   f"{gb / (avg * 1e-3):.1f} | {2 * cells / (avg * 1e-3) / 1e9:.2f} |")
+        gb = BPU[stype(name)] * cells / 1e9
+        out.append(f"| {name} | {lv} | {len(v)} | {avg:.4f} | {min(v):.4f} | {gb:.3f} | "
+                   f"{gb / (avg * 1e-3):.1f} | {lv * cells / (avg * 1e-3) / 1e9:.2f} |")
     return out
 
 

@@ -1,7 +1,7 @@
 """Batches: every channel of a batched launch evolves bit for bit as the same
-channel in its own context (and as the oracle), for one and two steps per
-launch, every launch shape and tile width, with and without hipGraph
-replay."""
+channel in its own context (and as the oracle), for one, two, four and six
+steps per launch, every launch shape, tile and chunk width, with and without
+hipGraph replay."""
 import numpy as np
 import pytest
 
@@ -153,11 +153,16 @@ def test_seeded_workload_and_autotune(tmp_path):
     with Batch(chans, "f32") as b:
         b.init_equilibrium()
         res = autotune(b, steps=4, graph_steps=2, ty_caps=(64, 256), wgs=(4, 8),
-                       tile_cols=(18,), panel_cols=(6, 8), cache_dir=str(tmp_path))
-        assert res["glups"] > 0 and len(res["candidates"]) == 7
-        assert [c["depth"] for c in res["candidates"]] == [1, 1, 1, 1, 2, 2, 2]
-        assert [c["sweep"] for c in res["candidates"]][-3:] == ["tight_block", "panel", "panel"]
-        assert [c["chunk_cols"] for c in res["candidates"]][-2:] == [6, 8]
+                       tile_cols=(18,), panel_cols=(6, 8), deep_cols=(8,),
+                       cache_dir=str(tmp_path))
+        assert res["glups"] > 0 and len(res["candidates"]) == 9
+        assert [c["depth"] for c in res["candidates"]] == [1, 1, 1, 1, 2, 2, 2, 2, 2]
+        assert [c["sweep"] for c in res["candidates"]][-5:] == [
+            "tight_block", "panel", "panel", "panel4", "panel6"]
+        assert [c["chunk_cols"] for c in res["candidates"]][-4:] == [6, 8, 8, 8]
+        assert [c["graph_steps"] for c in res["candidates"]][-3:] == [2, 12, 12]
+        assert res["steps_per_launch"] == {"panel4": 4, "panel6": 6}.get(
+            res["sweep"], res["depth"])
         again = autotune(b, cache_dir=str(tmp_path))
         assert again.get("cached") and again["ty_cap"] == res["ty_cap"]
         assert again["depth"] == res["depth"] and b.launch["depth"] == res["depth"]
@@ -166,14 +171,16 @@ def test_seeded_workload_and_autotune(tmp_path):
 
 
 def test_autotune_fp64_times_both_depths(tmp_path):
-    """The two-step candidates of FP64 widths; the state keeps advancing."""
+    """The multi-step candidates of FP64 widths; the state keeps advancing."""
     chans = channels_from_seed(16, seed=5)
     with Batch(chans, "f64") as b:
         b.init_equilibrium()
         res = autotune(b, steps=4, graph_steps=3, ty_caps=(64,), wgs=(4,), tile_cols=(8, 14),
-                       panel_cols=(6,), cache_dir=str(tmp_path))
-        assert [c["depth"] for c in res["candidates"]] == [1, 2, 2, 2]
-        assert res["candidates"][-1]["sweep"] == "panel"
+                       panel_cols=(6,), deep_cols=(12,), deep_graph_steps=6,
+                       cache_dir=str(tmp_path))
+        assert [c["depth"] for c in res["candidates"]] == [1, 2, 2, 2, 2, 2]
+        assert [c["sweep"] for c in res["candidates"]][-3:] == ["panel", "panel4", "panel6"]
+        assert res["graph_steps"] == (6 if res["sweep"] in ("panel4", "panel6") else 3)
         assert res["depth"] in (1, 2) and b.launch["depth"] == res["depth"]
         b.step(5)
         assert b.steps_done() > 0
@@ -218,6 +225,81 @@ def test_batch_register_march_chunk_widths(chunk_cols):
         for g, w, name in zip(got[i], want, ("lattice", "tmp", "outlet")):
             a, b_ = (g[2:ch.ly], w[2:ch.ly]) if name == "outlet" else (g[1:-1, 1:-1], w[1:-1, 1:-1])
             assert np.array_equal(a, b_), (i, name, chunk_cols)
+
+
+# channels with a deep interior for the four- and six-step marches (at least
+# 124 interior rows once the halo rows are off the walls)
+DEEP = [L.Channel(700, 250, 140, 125, 25, reynolds=90.0),
+        L.Channel(1030, 140, 200, 70, 20),
+        L.Channel(520, 300, 100, 150, 30, reynolds=300.0)]
+
+
+def assert_as_alone(got, chans, steps, storage, tag):
+    for i, ch in enumerate(chans):
+        want = alone(ch, steps, storage)
+        for g, w, name in zip(got[i], want, ("lattice", "tmp", "outlet")):
+            a, b_ = (g[2:ch.ly], w[2:ch.ly]) if name == "outlet" else (g[1:-1, 1:-1], w[1:-1, 1:-1])
+            assert np.array_equal(a, b_), (i, name, tag, np.abs(a - b_).max())
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32", "f16"])
+@pytest.mark.parametrize("sweep", ["panel4", "panel6"])
+@pytest.mark.parametrize("graph_steps", [0, 12])
+def test_batch_deep_march_equals_single_contexts(storage, sweep, graph_steps):
+    """The four- and six-step register marches over the channels with a deep
+    interior (interior waves in one launch, the frame in two-step passes and
+    the rings inside it through further buffers), two-step passes for the
+    channels without one, two-step launches for the remainders (35 steps:
+    one, then four-step launches and a two-step one, or six-step launches
+    and two two-step ones): every channel keeps the bits of the channel
+    alone, eagerly and replayed from hipGraphs."""
+    chans = small_channels() + DEEP
+    steps = 35
+    with Batch(chans, storage) as b:
+        base = b.footprint()
+        b.set_sweep(sweep)
+        # one (four steps) or two (six) further buffers of a pool's size
+        nmid = 1 if sweep == "panel4" else 2
+        assert abs((b.footprint() - base) - nmid * base / 2) < 0.01 * base
+        b.set_launch(0, 0, graph_steps)
+        b.init_equilibrium()
+        b.step(steps)
+        assert b.steps_done() == steps
+        got = [b.download(i) for i in range(len(chans))]
+    assert_as_alone(got, chans, steps, storage, (sweep, graph_steps))
+
+
+@pytest.mark.parametrize("chunk_cols", [4, 24, 64])
+@pytest.mark.parametrize("sweep", ["panel4", "panel6"])
+def test_batch_deep_march_chunk_widths(sweep, chunk_cols):
+    """Every deep chunk width cuts the deep channels into other interior
+    boxes, frames and rings: same bits."""
+    chans = small_channels()[:3] + DEEP
+    steps = 25
+    with Batch(chans, "f32") as b:
+        b.set_sweep(sweep, chunk_cols)
+        b.init_equilibrium()
+        b.step(steps)
+        got = [b.download(i) for i in range(len(chans))]
+    assert_as_alone(got, chans, steps, "f32", (sweep, chunk_cols))
+
+
+def test_batch_deep_march_matches_oracle(orc):
+    """Switching between every multi-step sweep of a batch, the deep marches
+    with remainders among them: every channel equals the oracle."""
+    from oracle_lib import Channel as OC
+    chans = small_channels()[:2] + DEEP[:2]
+    with Batch(chans, "f64") as b:
+        b.init_equilibrium()
+        for n, sweep in ((5, "panel4"), (7, "panel6"), (6, "tight_block"), (9, "panel4"),
+                         (4, "panel"), (13, "panel6")):
+            b.set_sweep(sweep)
+            b.step(n)
+        got = [b.download(i)[0] for i in range(len(chans))]
+    for ch, lat in zip(chans, got):
+        oc = OC(ch.lx, ch.ly, ch.obst_x, ch.obst_y, ch.obst_r, ch.u_max, ch.reynolds)
+        _, lref, _, _ = orc.run(oc, 44)
+        assert np.array_equal(lat[1:-1, 1:-1], lref[1:-1, 1:-1])
 
 
 def test_batch_sweep_switches_keep_results(orc):

@@ -2,8 +2,9 @@
 
 Batching: the 512 channels of seed 0 (lx log-uniform in [128, 2048],
 ly = lx/4, 96.9 M cells, 118 of them with tau < 0.505), advanced with
-10-step hipGraphs and both two-step sweeps (LDS tiles, register march), as
-the bench runs them.  The
+10-step hipGraphs and every multi-step sweep (two steps: LDS tiles, register
+march; four and six: the deep register marches with two-step remainders),
+as the bench runs them.  The
 smallest and the largest channel and those with tau < 0.505 equal the same
 channel alone in its own context bit for bit, in every storage format, and
 three channels (smallest, median, largest) equal the oracle (the reference's
@@ -25,7 +26,7 @@ from lbm_amd.serve import SolveServer, poisson_requests
 
 pytestmark = [pytest.mark.gpu, pytest.mark.slow]
 
-STEPS = 21      # one single step after initialisation, then ten two-step launches
+STEPS = 21      # one single step after initialisation, then two 10-step graphs
 
 
 @pytest.fixture(scope="module")
@@ -52,7 +53,7 @@ def test_seeded_batch_shape(seeded):
     assert chans[small].lx < 140 and chans[large].lx > 1900
 
 
-@pytest.mark.parametrize("sweep", ["tight_block", "panel"])
+@pytest.mark.parametrize("sweep", ["tight_block", "panel", "panel4", "panel6"])
 @pytest.mark.parametrize("storage", ["f64", "f32", "f16"])
 def test_seeded_batch_equals_single_contexts(seeded, storage, sweep):
     chans, small, _, large, tau = seeded
@@ -72,7 +73,7 @@ def test_seeded_batch_equals_single_contexts(seeded, storage, sweep):
             assert np.array_equal(a, b_), (storage, i, ch, name)
 
 
-@pytest.mark.parametrize("sweep", ["tight_block", "panel"])
+@pytest.mark.parametrize("sweep", ["tight_block", "panel", "panel4", "panel6"])
 def test_seeded_batch_matches_oracle(seeded, orc, sweep):
     from oracle_lib import Channel as OC
     chans, small, median, large, _ = seeded
