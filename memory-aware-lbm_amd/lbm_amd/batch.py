@@ -218,7 +218,7 @@ def batch_key(batch: Batch) -> str:
 
 def autotune(batch: Batch, steps: int = 20, graph_steps: int = 10,
              ty_caps=(32, 64, 128, 256), wgs=(2, 4, 8, 16), tile_cols=None,
-             panel_cols=(4, 6, 8), deep_cols=(12, 16, 24, 32, 48), deep_graph_steps: int = 12,
+             panel_cols=(4, 6, 8), deep_cols=(16, 24, 32, 48, 64), deep_graph_steps: int = 12,
              cache_dir: Optional[str] = TUNE_DIR, use_cache: bool = True,
              repeats: int = 2) -> dict:
     """Pick the launch shape with the highest throughput on this batch.

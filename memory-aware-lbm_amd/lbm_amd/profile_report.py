@@ -195,32 +195,36 @@ def march_levels(name: str) -> int:
 
 def batch_marches(trace_dir: str) -> Dict[Tuple[str, int], List[float]]:
     """Whole launches of the batched register marches, per (storage type,
-    steps): a two-step launch is a perimeter dispatch (k_batch_wedge, second
-    stream) and the next interior dispatch (k_batch_win<S, 2, 2>); a four- or
-    six-step launch is its frame passes (k_batch_frame) and the two-step
-    passes of its small channels on the second stream, then the interior
-    dispatch (k_batch_win<S, 2, 4 or 6>).  The dispatches of a launch run side
-    by side, so it lasts from the earliest start to the latest end (ms)."""
-    rows = sorted((int(r["Dispatch_Id"]), short(r["Kernel_Name"]), int(r["Start_Timestamp"]),
-                   int(r["End_Timestamp"])) for r in _rows(trace_dir, "kernel_trace.csv"))
+    steps).  A launch's dispatches (a two-step launch: k_batch_wedge on the
+    second stream beside k_batch_win<S, 2, 2>; a four- or six-step launch:
+    k_batch_win<S, 2, 4 or 6> beside its frame passes, k_batch_frame, and the
+    two-step passes of its small channels) overlap in time, and the next
+    launch starts only after all of them have ended (it forks from the join
+    of the last one), so a launch is a maximal run of time-overlapping
+    dispatches; it lasts from the earliest start to the latest end (ms) and
+    has as many steps as its interior dispatch with the most levels."""
+    rows = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), short(r["Kernel_Name"]))
+                  for r in _rows(trace_dir, "kernel_trace.csv"))
+    rows = [r for r in rows if r[2].startswith(("k_batch_wedge", "k_batch_win", "k_batch_frame"))]
     out: Dict[Tuple[str, int], List[float]] = defaultdict(list)
-    group: List[Tuple[int, int]] = []
-    deep = False
-    for _, name, t0, t1 in rows:
-        if name.startswith("k_batch_frame"):
-            if not deep:
-                group, deep = [], True
-            group.append((t0, t1))
-        elif name.startswith("k_batch_wedge"):
-            group.append((t0, t1))
-        elif name.startswith("k_batch_win"):
-            group.append((t0, t1))
-            lv = march_levels(name)
-            if lv == 2 and deep:
-                continue                      # a small channel's pass inside a deep launch
+
+    def close(group):
+        wins = [n for _, _, n in group if n.startswith("k_batch_win")]
+        if wins:
+            lv = max(march_levels(n) for n in wins)
             s, e = min(g[0] for g in group), max(g[1] for g in group)
-            out[(stype(name), lv)].append((e - s) * 1e-6)
-            group, deep = [], False
+            out[(stype(wins[0]), lv)].append((e - s) * 1e-6)
+
+    group: List[Tuple[int, int, str]] = []
+    end = 0
+    for t0, t1, name in rows:
+        if group and t0 >= end:
+            close(group)
+            group = []
+        group.append((t0, t1, name))
+        end = max(end, t1) if len(group) > 1 else t1
+    if group:
+        close(group)
     return out
 
 
