@@ -216,11 +216,20 @@ def batch_key(batch: Batch) -> str:
     return f"{batch.storage}-n{len(batch.channels)}-{h}"
 
 
+def _apply(batch: Batch, r: dict):
+    """Set a candidate's launch shape on the batch."""
+    batch.set_launch(r["wgs_per_cu"], r["ty_cap"], r["graph_steps"])
+    batch.set_depth(r["depth"], r["tile_cols"])
+    if r.get("panel_cols"):
+        batch.set_sweep("panel", r["panel_cols"])
+    batch.set_sweep(r.get("sweep", "tight_block"), r.get("chunk_cols", 0))
+
+
 def autotune(batch: Batch, steps: int = 20, graph_steps: int = 10,
              ty_caps=(32, 64, 128, 256), wgs=(2, 4, 8, 16), tile_cols=None,
              panel_cols=(4, 6, 8), deep_cols=(16, 24, 32, 48, 64), deep_graph_steps: int = 12,
              cache_dir: Optional[str] = TUNE_DIR, use_cache: bool = True,
-             repeats: int = 2) -> dict:
+             repeats: int = 2, finalists: int = 3) -> dict:
     """Pick the launch shape with the highest throughput on this batch.
 
     Every candidate runs `steps` steps `repeats` times (after one warm-up
@@ -231,7 +240,9 @@ def autotune(batch: Batch, steps: int = 20, graph_steps: int = 10,
     narrower than four columns), the two-step register march over
     panel_cols-column chunks, and the four- and six-step register marches
     over deep_cols-column chunks (hipGraphs of deep_graph_steps steps, a
-    multiple of both, timed over twice that).  Returns {"depth",
+    multiple of both, timed over twice that); the best `finalists` are
+    timed again over four graphs each and the best of those wins.  Returns
+    {"depth",
     "tile_cols", "ty_cap", "wgs_per_cu", "graph_steps", "sweep",
     "chunk_cols", "steps_per_launch", "glups", "candidates": [...]}; cached
     by batch_key."""
@@ -239,11 +250,7 @@ def autotune(batch: Batch, steps: int = 20, graph_steps: int = 10,
     path = os.path.join(cache_dir, key + ".json") if cache_dir else None
     if use_cache and path and os.path.exists(path):
         best = json.load(open(path))
-        batch.set_launch(best["wgs_per_cu"], best["ty_cap"], best["graph_steps"])
-        batch.set_depth(best.get("depth", 1), best.get("tile_cols", 0))
-        if best.get("panel_cols"):
-            batch.set_sweep("panel", best["panel_cols"])
-        batch.set_sweep(best.get("sweep", "tight_block"), best.get("chunk_cols", 0))
+        _apply(batch, best)
         best["cached"] = True
         return best
     cells = batch.cells
@@ -292,7 +299,20 @@ def autotune(batch: Batch, steps: int = 20, graph_steps: int = 10,
     for r in results:
         r.setdefault("sweep", "tight_block")
         r.setdefault("chunk_cols", 0)
-    best = max(results, key=lambda r: r["glups"])
+    # the short timings above rank candidates within a few percent of each
+    # other by noise: the best `finalists` run again over 4 graphs each, and
+    # the best of those longer timings wins
+    ranked = sorted(results, key=lambda r: -r["glups"])[:max(1, finalists)]
+    if len(ranked) > 1:
+        for r in ranked:
+            _apply(batch, r)
+            gs = r["graph_steps"]
+            batch.time_steps(gs)
+            ms = min(batch.time_steps(4 * gs) for _ in range(max(1, repeats)))
+            r["glups_final"] = cells * 4 * gs / (ms * 1e-3) / 1e9
+        best = max(ranked, key=lambda r: r["glups_final"])
+    else:
+        best = ranked[0]
     spl = 1 if best["depth"] < 2 else {"panel4": 4, "panel6": 6}.get(best["sweep"], 2)
     out = {"depth": best["depth"], "tile_cols": best["tile_cols"], "ty_cap": best["ty_cap"],
            "wgs_per_cu": best["wgs_per_cu"], "graph_steps": best["graph_steps"],
@@ -300,9 +320,7 @@ def autotune(batch: Batch, steps: int = 20, graph_steps: int = 10,
            "panel_cols": best.get("panel_cols", 0), "steps_per_launch": spl,
            "glups": best["glups"], "key": key,
            "tuned_at": time.strftime("%Y-%m-%d %H:%M:%S"), "candidates": results}
-    batch.set_launch(out["wgs_per_cu"], out["ty_cap"], out["graph_steps"])
-    batch.set_depth(out["depth"], out["tile_cols"])
-    batch.set_sweep(out["sweep"], out["chunk_cols"])
+    _apply(batch, out)
     if path:
         os.makedirs(cache_dir, exist_ok=True)
         with open(path, "w") as fh:

@@ -52,3 +52,33 @@ def test_passes_stay_within_one_run_limits():
         assert sum(c.startswith("TCC_") for c in p) <= 4
     collected = {c for p in counters.PASSES for c in p}
     assert {c for ev in counters.EVENTS for c in ev} <= collected
+
+
+def test_batch_launches_are_time_overlap_groups(tmp_path):
+    """The profile report's batch launches: dispatches that overlap in time
+    form one launch (whatever their submission order: the deep launch
+    enqueues its interior before its frame), a launch has the steps of its
+    deepest interior dispatch, and the next launch starts after the last one
+    ended."""
+    from lbm_amd import profile_report as pr
+    ms = 1_000_000
+    rows = [  # (dispatch id, kernel, start, end) in ns
+        (1, "void lbm::k_batch_win<double, 2, 6>(lbm::ChannelWaves const*)", 0, 6 * ms),
+        (2, "void lbm::k_batch_frame<double, 16>(lbm::ChannelWaves const*)", 10, 1 * ms),
+        (3, "void lbm::k_batch_frame<double, 8>(lbm::ChannelWaves const*)", 1 * ms, 2 * ms),
+        (4, "void lbm::k_batch_wedge<double, 4>(lbm::ChannelWaves const*)", 2 * ms, 3 * ms),
+        (5, "void lbm::k_batch_win<double, 2, 2>(lbm::ChannelWaves const*)", 3 * ms, 4 * ms),
+        (6, "lbm::k_bump(unsigned int*, unsigned int)", 6 * ms, 6 * ms + 10),
+        (7, "void lbm::k_batch_win<double, 2, 6>(lbm::ChannelWaves const*)", 7 * ms, 12 * ms),
+        (8, "void lbm::k_batch_frame<double, 16>(lbm::ChannelWaves const*)", 7 * ms, 13 * ms),
+        (9, "void lbm::k_batch_wedge<double, 4>(lbm::ChannelWaves const*)", 14 * ms, 15 * ms),
+        (10, "void lbm::k_batch_win<double, 2, 2>(lbm::ChannelWaves const*)", 14 * ms, 17 * ms),
+    ]
+    with open(tmp_path / "run_kernel_trace.csv", "w") as fh:
+        fh.write("Dispatch_Id,Kernel_Name,Start_Timestamp,End_Timestamp\n")
+        for d, k, t0, t1 in rows:
+            fh.write(f'{d},"{k}",{t0},{t1}\n')
+    got = pr.batch_marches(str(tmp_path))
+    assert sorted(got) == [("double", 2), ("double", 6)]
+    assert [round(v, 6) for v in got[("double", 6)]] == [6.0, 6.0]
+    assert [round(v, 6) for v in got[("double", 2)]] == [3.0]
