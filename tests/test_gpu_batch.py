@@ -284,6 +284,35 @@ def test_batch_deep_march_chunk_widths(sweep, chunk_cols):
     assert_as_alone(got, chans, steps, "f32", (sweep, chunk_cols))
 
 
+def test_batch_deep_sweep_settings():
+    """Odd or out-of-range chunk widths are refused and leave the sweep as it
+    was; a batch holding a channel narrower than four columns refuses the
+    register marches; a batch whose channels are all too small for the deep
+    march still runs LBM_PANEL4 (as two-step passes) with the same bits."""
+    chans = small_channels()[:3] + DEEP[:1]
+    with Batch(chans, "f64") as b:
+        b.set_sweep("panel6", 32)
+        for bad in (5, 2, 66):
+            with pytest.raises(L.LbmError, match="chunk_cols"):
+                b.set_sweep("panel6", bad)
+        b.init_equilibrium()
+        b.step(13)
+        got = [b.download(i) for i in range(len(chans))]
+    assert_as_alone(got, chans, 13, "f64", "after refused settings")
+    with Batch([L.Channel(3, 8, 1, 4, 1)] + DEEP[:1], "f64") as b:
+        with pytest.raises(L.LbmError):
+            b.set_sweep("panel4")
+    # fewer than 12 rows: below the four-step frame's two widened strips
+    small = [L.Channel(17, 5, 4, 2, 1, reynolds=40.0), L.Channel(9, 4, 2, 2, 1),
+             L.Channel(40, 10, 8, 5, 2)]
+    with Batch(small, "f32") as b:
+        b.set_sweep("panel4", 16)
+        b.init_equilibrium()
+        b.step(11)
+        got = [b.download(i) for i in range(len(small))]
+    assert_as_alone(got, small, 11, "f32", "all shallow")
+
+
 def test_batch_deep_march_matches_oracle(orc):
     """Switching between every multi-step sweep of a batch, the deep marches
     with remainders among them: every channel equals the oracle."""
