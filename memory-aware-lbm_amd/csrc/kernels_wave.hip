@@ -16,8 +16,7 @@
 // Nothing goes through LDS.  Four steps stack three such levels (58 owned
 // rows, three halo rows each side), six steps five (54 owned rows).  The
 // deep sweeps' perimeter frame advances two steps per pass through further
-// buffers, each pass but the last also filling a ring a few nodes deep just
-// inside the interior rectangle for the next pass (k_ring2).
+// buffers, each pass but the last widened inwards by what the next one reads.
 //
 // Waves whose rows or columns touch the channel perimeter run a separate,
 // general kernel (k_wpanel_edge: wall operators, outlet extrapolation,
@@ -564,19 +563,13 @@ WaveGrid wave_grid(const Layout &L, int cx) {
     return make_thin(L, wave_grid_max(L, cx, 16, 16384), 1, WAVE_ROWS);
 }
 
-// The grid of the deep sweeps (LEVELS = 4, 6 or 8 steps per launch, H = LEVELS-1
-// halo rows and columns): the interior chunks of the two-step grid, shrunk
-// until their H-column halo stays off columns 1 and lx, over rows r0 = H + 2
-// .. r1 = ly - H, the rows whose H-row halo stays within rows 2 .. ly - 1,
-// tiled by (64 - 2H)-row bands (the last one partial).  The frame is then
-// H + 1 rows deep along the walls: its perimeter waves take eight chunks of
-// such a strip each (eight lanes per chunk), so a pass has few perimeter
-// waves: 196 on 16384 x 4096 against 896 for a frame of whole 62-row bands
-// (the two-step grid, sixteen chunks of its two-row strips per wave: 260
-// against 2178).  Perimeter waves beyond about one
-// per CU keep the interior kernel's workgroups off their CUs; with the thin
-// frame the six-step FP64 sweep runs 2-3% faster, the four-step FP32 / FP16
-// ones 3-6%, the two-step FP32 / FP16 ones 3-4%.  The deep
+// The thin-frame grid of the deep sweeps (LEVELS = 4, 6 or 8 steps per
+// launch, H = LEVELS-1 halo rows and columns): the interior chunks of the
+// two-step grid, shrunk until their H-column halo stays off columns 1 and lx,
+// over rows r0 = H + 2 .. r1 = ly - H, tiled by (64 - 2H)-row bands.  It
+// decides whether a lattice takes a deep sweep (wpanel_deep_possible) and the
+// chunk width; the launch then runs on the aligned grid over the same rows
+// (aligned_deep_grid).  The deep
 // march recomputes 2H halo columns per chunk and does LEVELS/2 times the
 // two-step arithmetic per column, so it wants wider chunks, up to 24, and
 // fewer waves (about 6144) before narrowing them: four-step FP64 GLUPS at 4 /
@@ -596,8 +589,7 @@ static WaveGrid wave_grid_deep(const Layout &L, int cx, int storage, int levels)
     const bool largest = bands * L.lx >= 128 * 6144;
     const int cmax = !largest ? 24 : levels >= 6 ? 64 : storage != 2 ? 40 : 24;
     if (cx < 3 && largest && levels == 4 && storage == 0) cx = 63;
-    // at least two bands of interior rows: they hold the rings of the
-    // perimeter passes, at most LEVELS - 2 nodes thick, many times over
+    // at least two bands of interior rows
     return make_thin(L, wave_grid_max(L, cx, cmax, 6144), levels - 1, 2 * WAVE_ROWS);
 }
 
@@ -665,104 +657,133 @@ hipError_t launch_wpanel(int storage, const Problem &p, const WaveGrid &g, const
     return e;
 }
 
-// ---- rings of a few nodes inside the deep interior ---------------------------
-// The perimeter passes of a deep launch run on the frame around the deep
-// interior rectangle; each pass but the last must also leave its output on a
-// ring of t nodes just inside the rectangle, where the next pass's perimeter
-// waves (their halo lanes and columns compute a first step one node in, from
-// pulls two nodes in) and the next pass's ring pull from.  These few rows
-// and columns are interior nodes (the rectangle's halo stays off the walls);
-// one thread per ring node recomputes the first step of its 3 x 3
-// neighbourhood from the pass's input and then the second step of the node,
-// the same operations as every other kernel, so the same bits.
-// Ring node k of r.
-template <typename S>
-__device__ __forceinline__ void ring2_node(const Problem &p, const Ring &r, int k, uint32_t dstep,
+// ---- the frame of the deep marches ---------------------------------------------
+// The interior waves of a deep launch cover rows [H + 2, ly - H) and columns
+// [H + 2, lx - H) (H = LEVELS - 1: their halo stays off the walls), in
+// cx-column chunks right-aligned at lx - H, the leftmost moved right to start
+// at H + 2 (it overlaps its neighbour, which writes the same values).  The
+// frame around them advances two steps per pass (LEVELS / 2 passes through
+// LEVELS / 2 - 1 further buffers); each pass but the last is widened inwards
+// by the depth its successor reads (DeepFrame), so every pass reads only what
+// the one before it wrote and no other kernel is needed.
+
+// lanes per strip chunk of a frame pass with sr strip rows (and two halo rows)
+constexpr int frame_gl(int sr) { return sr + 2 <= 8 ? 8 : 16; }
+
+// columns per strip chunk of a batched deep march's frame
+constexpr int FRAME_CX = 16;
+
+// strip rows of pass j of a LEVELS-step frame (0 past the last pass)
+constexpr int frame_rows(int levels, int j) {
+    const int nmid = levels / 2 - 1;
+    return j > nmid ? 0 : levels + (j < nmid ? ring_depth(nmid, j) : 0);
+}
+
+// Perimeter wave e of a pass of a batched deep march's frame (DeepFrame)
+// with sr strip rows, nsw waves per strip and nsb side bands: the bottom
+// strip's waves, the top strip's, then the left side's bands and the right
+// side's.  GL: lanes per strip chunk (sr + 2 <= GL).
+template <typename S, int GL>
+__device__ __forceinline__ void frame_wave(const Problem &p, const WaveGrid &sg, int sr, int nsw,
+                                           int nsb, int e, uint32_t dstep,
                                            const S *__restrict__ P, S *__restrict__ Q) {
-    using T = typename Calc<S>::T;
     const Layout &L = p.lay;
-    if (k >= r.n) return;
-    // y runs fastest (neighbouring threads read neighbouring elements): the
-    // bottom and top rows column by column, then the left and right columns
-    const int w = r.xb - r.xa, band = r.t * w, hs = r.yb - r.ya - 2 * r.t;
-    int x, y;
-    if (k < band) {
-        x = r.xa + k / r.t;
-        y = r.ya + k % r.t;
-    } else if (k < 2 * band) {
-        x = r.xa + (k - band) / r.t;
-        y = r.yb - r.t + (k - band) % r.t;
-    } else {
-        const int j = k - 2 * band, c = j / hs;
-        y = r.ya + r.t + j % hs;
-        x = c < r.t ? r.xa + c : r.xb - 2 * r.t + c;
+    if (e < 2 * nsw) {
+        const bool top = e >= nsw;
+        const int ck0 = (top ? e - nsw : e) * (64 / GL);
+        edge_march<S, GL>(p, sg, dstep, P, Q, top ? L.ly + 1 - sr : 1, sr, ck0);
+        return;
     }
-    const uint32_t m = sizeof(S) == 4 ? *p.step + dstep : 0u;
-    if constexpr (sizeof(S) == 4) fp32_round_toward_zero();   // no integer division below
-    T h[9];
-    for_pops([&]<int I>() {
-        // first step of the neighbour this population streams from
-        const int nx = x - cx(I), ny = y - cy(I);
-        const int64_t n = L.node(nx, ny);
-        T g[9];
-        for_pops([&]<int J>() {
-            g[J] = as_calc<S>(P[J * L.plane + n - cx(J) * L.pitch - cy(J)]);
-        });
-        collide_any<S>(p, -1, nx, ny, g, 0., m);
-        h[I] = stored<S>(g[I]);
-    });
-    collide_any<S>(p, -1, x, y, h, 0., m + 1);
-    const int64_t n = L.node(x, y);
-    for_pops([&]<int I>() { Q[I * L.plane + n] = to_storage<S>(h[I]); });
+    e -= 2 * nsw;
+    const bool right = e >= nsb;
+    const int Y0 = 1 + sr + (right ? e - nsb : e) * WAVE_ROWS;
+    const int Y1 = L.ly + 1 - sr;
+    edge_march<S, 64>(p, sg, dstep, P, Q, Y0, Y1 - Y0 < WAVE_ROWS ? Y1 - Y0 : WAVE_ROWS, 0,
+                      right ? L.lx + 1 - sr : 1, right ? L.lx + 1 : 1 + sr);
 }
 
-template <typename S>
-__global__ __launch_bounds__(256) void k_ring2(Problem p, Ring r, uint32_t dstep,
-                                               const S *__restrict__ P, S *__restrict__ Q) {
-    ring2_node<S>(p, r, (int)(blockIdx.x * blockDim.x + threadIdx.x), dstep, P, Q);
+template <typename S, int GL>
+__global__ __launch_bounds__(64) void k_frame(Problem p, WaveGrid sg, int sr, int nsw, int nsb,
+                                              uint32_t dstep, const S *__restrict__ P,
+                                              S *__restrict__ Q) {
+    frame_wave<S, GL>(p, sg, sr, nsw, nsb, (int)blockIdx.x, dstep, P, Q);
 }
 
-// the deep interior rectangle of grid g
-static Ring deep_rect(const Layout &L, const WaveGrid &g) {
-    Ring r;
-    r.xa = L.lx + 1 - (g.ncx - g.c_lo) * g.cx;
-    r.xb = L.lx + 1 - (g.ncx - g.c_hi) * g.cx;
-    r.ya = g.r0;
-    r.yb = g.r1;
-    r.t = r.n = 0;
-    return r;
+// The interior grid and frame of a deep march of LEVELS = 4, 6 or 8 steps,
+// interior chunks of cx columns, strips in chunks of strip_cx; false when the
+// lattice has less than one chunk of interior columns or one interior row,
+// or its widest strips or sides meet.
+bool aligned_deep_grid(const Layout &L, int cx, int strip_cx, int levels, WaveGrid *g,
+                       DeepFrame *f) {
+    if ((levels != 4 && levels != 6 && levels != 8) || cx < 3 || !wave_grid_fits(L)) return false;
+    const int H = levels - 1, nmid = levels / 2 - 1;
+    const int xl = H + 2, xr = L.lx - H, r0 = H + 2, r1 = L.ly - H;
+    const int sr0 = frame_rows(levels, 0);
+    if (xr - xl < cx || r1 - r0 < 1 || 2 * sr0 > L.ly || 2 * sr0 > L.lx) return false;
+    // 32-bit offsets of the interior kernel's buffer resources
+    if ((int64_t)(cx + 16) * L.pitch * 8 >= (int64_t)DROP) return false;
+    WaveGrid w = wave_grid_max(L, cx, cx, 1);
+    f->sg = wave_grid_max(L, strip_cx, strip_cx, 1);
+    f->nmid = nmid;
+    for (int j = 0; j < 4; ++j) {
+        const int sr = frame_rows(levels, j);
+        f->sr[j] = sr;
+        f->nsw[j] = j <= nmid ? (f->sg.ncx + 64 / frame_gl(sr) - 1) / (64 / frame_gl(sr)) : 0;
+        f->nsb[j] = j <= nmid ? (L.ly - 2 * sr + WAVE_ROWS - 1) / WAVE_ROWS : 0;
+    }
+    w.xl = xl;
+    w.xr = xr;
+    w.ncx = (xr - xl + cx - 1) / cx;
+    w.c_lo = 0;
+    w.c_hi = w.ncx;
+    w.r0 = r0;
+    w.r1 = r1;
+    w.nbl = (r1 - r0 + 64 - 2 * H - 1) / (64 - 2 * H);
+    w.nin = w.nbl * w.ncx;
+    w.b_lo = w.b_hi = 0;
+    w.thin = 1;
+    w.nthin = w.nedge = 0;
+    w.gl = 0;
+    *g = w;
+    return true;
 }
 
+// perimeter waves of pass j
+int batch_frame_waves(const DeepFrame &f, int j) { return 2 * f.nsw[j] + 2 * f.nsb[j]; }
 
 hipError_t launch_wpanel_deep(int storage, const Problem &p, int cx, int levels, const void *P,
                               void *const *mid, void *Q, hipStream_t st, hipStream_t side,
                               hipEvent_t fork, hipEvent_t join, KernelProbe *probe) {
     if (!wpanel_deep_possible(p.lay, cx, storage, levels)) return hipErrorInvalidValue;
     const Layout &L = p.lay;
-    const WaveGrid g = wave_grid_deep(L, cx, storage, levels);
+    // the chunk width of the thin-frame grid (its heuristics), the aligned grid
+    // and frame over it, the strips in chunks of that width too
+    const int ccx = wave_grid_deep(L, cx, storage, levels).cx;
+    WaveGrid g;
+    DeepFrame f;
+    if (!aligned_deep_grid(L, ccx, ccx, levels, &g, &f)) return hipErrorInvalidValue;
     const int nmid = levels / 2 - 1;
     // two columns of loads in flight when the chunk width allows, except for
     // the eight-step FP64 march (two columns in flight need 265 registers,
     // one wave per SIMD; one column 252, two waves)
     const int depth = g.cx % 2 == 0 && !(levels == 8 && storage == 0) ? 2 : 1;
-    const Ring ring = deep_rect(L, g);
     hipError_t e = hipEventRecord(fork, st);
     if (e == hipSuccess) e = hipStreamWaitEvent(side, fork, 0);
     if (e != hipSuccess) return e;
     auto go = [&](auto tag) {
         using T = decltype(tag);
         // perimeter pass j advances the frame by steps 2j+1, 2j+2, from P or
-        // mid[j-1] into mid[j] or Q; all but the last also fill a ring inside
-        // the interior rectangle (ring_depth)
+        // mid[j-1] into mid[j] or Q
         for (int j = 0; j <= nmid; ++j) {
             const T *src = (const T *)(j == 0 ? P : mid[j - 1]);
             T *dst = (T *)(j == nmid ? Q : mid[j]);
-            if (g.gl == 16) k_wpanel_edge<T, 16><<<g.nedge, 64, 0, side>>>(p, g, 2u * j, src, dst);
-            else k_wpanel_edge<T, 8><<<g.nedge, 64, 0, side>>>(p, g, 2u * j, src, dst);
-            if (j < nmid) {
-                const Ring r = ring_of(ring, ring_depth(nmid, j));
-                k_ring2<T><<<(r.n + 255) / 256, 256, 0, side>>>(p, r, 2u * j, src, dst);
-            }
+            const int n = batch_frame_waves(f, j);
+            if (frame_gl(f.sr[j]) == 8)
+                k_frame<T, 8><<<n, 64, 0, side>>>(p, f.sg, f.sr[j], f.nsw[j], f.nsb[j], 2u * j,
+                                                  src, dst);
+            else
+                k_frame<T, 16><<<n, 64, 0, side>>>(p, f.sg, f.sr[j], f.nsw[j], f.nsb[j], 2u * j,
+                                                   src, dst);
         }
         const T *src = (const T *)P;
         T *dst = (T *)Q;
@@ -789,13 +810,12 @@ hipError_t launch_wpanel_deep(int storage, const Problem &p, int cx, int levels,
 // ---- many channels per launch ----------------------------------------------------
 // The batched forms of the register marches: every channel of a batch has its
 // own wave grid (ChannelWaves); one launch runs a pass of every channel's
-// perimeter waves, one every channel's interior waves, one every channel's
-// ring, from flat work tables.  An interior workgroup's waves belong to one
+// perimeter waves, one every channel's interior waves, from flat work
+// tables.  An interior workgroup's waves belong to one
 // channel (their lockstep barriers need the same march), and the workgroups
 // of a channel are contiguous in the table, so xcd_block keeps a channel's
-// neighbouring bands in one L2.  Each wave and ring thread runs the
-// single-lattice sweep's own code, so a channel evolves bit for bit as it
-// would alone.
+// neighbouring bands in one L2.  Each wave runs the single-lattice sweep's
+// own code, so a channel evolves bit for bit as it would alone.
 
 template <typename S, int GL>
 __global__ __launch_bounds__(64) void k_batch_wedge(const ChannelWaves *__restrict__ ch,
@@ -816,29 +836,6 @@ __global__ __launch_bounds__((64 * wpg_of<S, DEPTH, LEVELS>())) void k_batch_win
     const ChannelWaves c = ch[__builtin_amdgcn_readfirstlane(it.x)];
     const int t = __builtin_amdgcn_readfirstlane(it.y * WPG + (int)(threadIdx.x >> 6));
     wpanel_in_wave<S, DEPTH, LEVELS>(c.p, c.g, t, dstep, A + c.off, B + c.off);
-}
-
-// Perimeter wave e of a pass of a batched deep march's frame (DeepFrame)
-// with sr strip rows, nsw waves per strip and nsb side bands: the bottom
-// strip's waves, the top strip's, then the left side's bands and the right
-// side's.  GL: lanes per strip chunk (sr + 2 <= GL).
-template <typename S, int GL>
-__device__ __forceinline__ void frame_wave(const Problem &p, const WaveGrid &sg, int sr, int nsw,
-                                           int nsb, int e, uint32_t dstep,
-                                           const S *__restrict__ P, S *__restrict__ Q) {
-    const Layout &L = p.lay;
-    if (e < 2 * nsw) {
-        const bool top = e >= nsw;
-        const int ck0 = (top ? e - nsw : e) * (64 / GL);
-        edge_march<S, GL>(p, sg, dstep, P, Q, top ? L.ly + 1 - sr : 1, sr, ck0);
-        return;
-    }
-    e -= 2 * nsw;
-    const bool right = e >= nsb;
-    const int Y0 = 1 + sr + (right ? e - nsb : e) * WAVE_ROWS;
-    const int Y1 = L.ly + 1 - sr;
-    edge_march<S, 64>(p, sg, dstep, P, Q, Y0, Y1 - Y0 < WAVE_ROWS ? Y1 - Y0 : WAVE_ROWS, 0,
-                      right ? L.lx + 1 - sr : 1, right ? L.lx + 1 : 1 + sr);
 }
 
 // Pass j's per-channel counts are read from the table in global memory: a
@@ -875,63 +872,15 @@ int batch_deep_wpg(int storage, int levels) {
                                                                 : wpg_of<_Float16, 2, 6>();
 }
 
-// lanes per strip chunk of a frame pass with sr strip rows (and two halo rows)
-constexpr int frame_gl(int sr) { return sr + 2 <= 8 ? 8 : 16; }
-
-// columns per strip chunk of a batched deep march's frame
-constexpr int FRAME_CX = 16;
-
-// strip rows of pass j of a LEVELS-step frame (0 past the last pass)
-constexpr int frame_rows(int levels, int j) {
-    const int nmid = levels / 2 - 1;
-    return j > nmid ? 0 : levels + (j < nmid ? ring_depth(nmid, j) : 0);
-}
-
-// The interior grid and frame of a batched deep march (DeepFrame): interior
-// rows [H + 2, ly - H) in (64 - 2H)-row bands, columns [H + 2, lx - H) in
-// cx-column chunks right-aligned at lx - H (the first one moved right to
-// start at H + 2); false when the channel has less than one chunk of
-// interior columns or one interior row, or its widest strips or sides meet.
+// The interior grid and frame of a batched deep march: LEVELS = 4 or 6, even
+// chunk widths (two columns of loads in flight), strips in FRAME_CX-column
+// chunks whatever the interior's width: shorter marches, more waves (512
+// channels, six steps in 64-column chunks: FP64 91.9 -> 97.1 GLUPS with
+// 16-column strip chunks, 96.8 with 8; FP32 99.3 -> 100.7).
 bool batch_deep_grid(const Layout &L, int cx, int levels, WaveGrid *g, DeepFrame *f) {
-    if ((levels != 4 && levels != 6) || cx < 4 || cx % 2 || !wave_grid_fits(L)) return false;
-    const int H = levels - 1, nmid = levels / 2 - 1;
-    const int xl = H + 2, xr = L.lx - H, r0 = H + 2, r1 = L.ly - H;
-    const int sr0 = frame_rows(levels, 0);
-    if (xr - xl < cx || r1 - r0 < 1 || 2 * sr0 > L.ly || 2 * sr0 > L.lx) return false;
-    // 32-bit offsets of the interior kernel's buffer resources
-    if ((int64_t)(cx + 16) * L.pitch * 8 >= (int64_t)DROP) return false;
-    WaveGrid w = wave_grid_max(L, cx, cx, 1);
-    // strips in chunks of FRAME_CX columns, whatever the interior's width:
-    // shorter marches, more waves (512 channels, six steps in 64-column
-    // chunks: FP64 91.9 -> 97.1 GLUPS with 16-column strip chunks, 96.8 with
-    // 8; FP32 99.3 -> 100.7)
-    f->sg = wave_grid_max(L, FRAME_CX, FRAME_CX, 1);
-    f->nmid = nmid;
-    for (int j = 0; j < 3; ++j) {
-        const int sr = frame_rows(levels, j);
-        f->sr[j] = sr;
-        f->nsw[j] = j <= nmid ? (f->sg.ncx + 64 / frame_gl(sr) - 1) / (64 / frame_gl(sr)) : 0;
-        f->nsb[j] = j <= nmid ? (L.ly - 2 * sr + WAVE_ROWS - 1) / WAVE_ROWS : 0;
-    }
-    w.xl = xl;
-    w.xr = xr;
-    w.ncx = (xr - xl + cx - 1) / cx;
-    w.c_lo = 0;
-    w.c_hi = w.ncx;
-    w.r0 = r0;
-    w.r1 = r1;
-    w.nbl = (r1 - r0 + 64 - 2 * H - 1) / (64 - 2 * H);
-    w.nin = w.nbl * w.ncx;
-    w.b_lo = w.b_hi = 0;
-    w.thin = 1;
-    w.nthin = w.nedge = 0;
-    w.gl = 0;
-    *g = w;
-    return true;
+    if ((levels != 4 && levels != 6) || cx < 4 || cx % 2) return false;
+    return aligned_deep_grid(L, cx, FRAME_CX, levels, g, f);
 }
-
-// perimeter waves of pass j
-int batch_frame_waves(const DeepFrame &f, int j) { return 2 * f.nsw[j] + 2 * f.nsb[j]; }
 
 // one two-step pass of the tables' channels (perimeter then interior waves,
 // both on `q`: they write disjoint nodes)

@@ -118,51 +118,28 @@ hipError_t launch_wpanel(int storage, const Problem &p, const WaveGrid &g, const
                          hipStream_t st, hipStream_t side, hipEvent_t fork, hipEvent_t join,
                          KernelProbe *probe = nullptr);
 // LEVELS = 4, 6 or 8 steps per launch: the interior waves march LEVELS levels
-// in registers (P -> Q); the perimeter frame runs LEVELS/2 two-step launches
-// through LEVELS/2 - 1 further buffers mid[] (P -> mid[0] over the frame
-// widened by LEVELS/2 - 1 bands / chunks, ..., the last one into Q).
-// Returns hipErrorInvalidValue when the lattice has no interior for it.
+// in registers (P -> Q); the perimeter frame (DeepFrame) runs LEVELS/2
+// two-step launches through LEVELS/2 - 1 further buffers mid[] (P -> mid[0]
+// -> ... -> Q).  Returns hipErrorInvalidValue when the lattice has no
+// interior for it (wpanel_deep_possible: an interior of at least 124 rows
+// and two chunks of the thin-frame grid).
 bool wpanel_deep_possible(const Layout &L, int cx, int storage, int levels);
 hipError_t launch_wpanel_deep(int storage, const Problem &p, int cx, int levels, const void *P,
                               void *const *mid, void *Q, hipStream_t st, hipStream_t side,
                               hipEvent_t fork, hipEvent_t join, KernelProbe *probe = nullptr);
 
-// The deep interior rectangle [xa, xb) x [ya, yb) of a deep grid and a ring
-// of t nodes just inside it (n nodes), which a deep launch's perimeter passes
-// but the last also fill.
-struct Ring {
-    int xa, xb, ya, yb;
-    int t;
-    int n;
-};
-
-// the ring of rectangle r, t nodes thick
-__host__ __device__ inline Ring ring_of(Ring r, int t) {
-    r.t = t;
-    const int wx = r.xb - r.xa, wy = r.yb - r.ya;
-    r.n = 2 * t * wx + 2 * t * (wy - 2 * t);
-    return r;
-}
-
-// thickness of the ring that perimeter pass j (of nmid + 1) fills: 2 nodes
-// for the pass before the last (the perimeter waves' halo computes a first
-// step one node in, from pulls two nodes in), 2 deeper for each pass before
-// that (a ring node t-1 deep takes its neighbour's first step t deep, from
-// pulls t+1 deep)
+// how much deeper than the last pass's frame perimeter pass j (of nmid + 1)
+// reaches into the interior: 2 nodes for the pass before the last (the next
+// pass's halo computes a first step one node in, from pulls two nodes in), 2
+// deeper for each pass before that
 __host__ __device__ constexpr int ring_depth(int nmid, int j) { return 2 + 2 * (nmid - 1 - j); }
 
-// Many channels of a batch (pool offset `off` elements), several steps per
-// launch with the register marches over each channel's wave grid.  The
-// channels' P and Q sit at A + off and B + off.  A work table lists
-// (channel, wave) for perimeter waves, (channel, first wave / WPG) for the
-// workgroups of interior waves (the workgroups of a channel contiguous) and
-// (channel, first node / 256) for ring nodes.
-// The frame of a batched deep march (LEVELS = 2 (nmid + 1) steps, H = LEVELS
+// The frame of a deep march (LEVELS = 2 (nmid + 1) steps, H = LEVELS
 // - 1): its interior waves cover rows [H + 2, ly - H) and columns
 // [H + 2, lx - H); perimeter pass j (steps 2j + 1, 2j + 2) covers the
 // frame around them widened inwards by the depth t_j its successor reads
 // (t_nmid = 0, t_j = t_{j+1} + 2: the next pass's halo computes a first step
-// one node further in, from pulls two nodes in), so no ring kernel is needed:
+// one node further in, from pulls two nodes in), so no other kernel is needed:
 // strips of rows [1, 1 + sr_j) and [ly + 1 - sr_j, ly + 1) over the whole
 // width (chunks of the strip grid sg, 64 / gl per wave, gl = 8 lanes per
 // chunk for sr_j + 2 <= 8, else 16), and the columns
@@ -170,12 +147,17 @@ __host__ __device__ constexpr int ring_depth(int nmid, int j) { return 2 + 2 * (
 // 62-row bands, with sr_j = H + 1 + t_j.
 struct DeepFrame {
     int nmid;            // passes - 1
-    int sr[3];           // strip rows (= side columns) of each pass
-    int nsw[3];          // waves per strip of each pass
-    int nsb[3];          // side bands of each pass, per side
+    int sr[4];           // strip rows (= side columns) of each pass
+    int nsw[4];          // waves per strip of each pass
+    int nsb[4];          // side bands of each pass, per side
     WaveGrid sg;         // the strips' chunks (cx, ncx; right-aligned at lx + 1)
 };
 
+// Many channels of a batch (pool offset `off` elements), several steps per
+// launch with the register marches over each channel's wave grid.  The
+// channels' P and Q sit at A + off and B + off.  A work table lists
+// (channel, wave) for perimeter waves and (channel, first wave / WPG) for the
+// workgroups of interior waves (the workgroups of a channel contiguous).
 struct ChannelWaves {
     Problem p;
     WaveGrid g;
@@ -201,6 +183,10 @@ hipError_t launch_batch_wpanel(int storage, const BatchTables &t, uint32_t dstep
 // two-step grids: LEVELS/2 two-step launches on `side` through the same
 // buffers).  batch_deep_grid: false when the channel is too small for it.
 bool batch_deep_grid(const Layout &L, int cx, int levels, WaveGrid *g, DeepFrame *f);
+// the grid and frame of a deep march of LEVELS = 4, 6 or 8 steps in cx-column
+// chunks, strips in strip_cx-column chunks (false: the lattice is too small)
+bool aligned_deep_grid(const Layout &L, int cx, int strip_cx, int levels, WaveGrid *g,
+                       DeepFrame *f);
 int batch_frame_waves(const DeepFrame &f, int j);
 int batch_deep_wpg(int storage, int levels);
 hipError_t launch_batch_wpanel_deep(int storage, int levels, const BatchTables &deep,

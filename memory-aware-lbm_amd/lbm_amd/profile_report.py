@@ -15,9 +15,9 @@ FP32, 36 FP16 -- times the nodes the dispatch updates once, i.e. one read and
 one write of every population) where the kernel carries a sweep's updates,
 and the measured HBM bytes per dispatch: FETCH_SIZE and WRITE_SIZE (KiB)
 scaled by what each reads for the plain copy kernel k_copy, whose bytes are
-known.  The panel sweeps launch their perimeter (k_wpanel_edge) and ring
-(k_ring2) kernels on a second stream before the interior kernel
-(k_wpanel_in); each such dispatch is attributed to the next interior
+known.  The panel sweeps launch their perimeter kernels (k_wpanel_edge; the
+deep sweeps' frame passes, k_frame) on a second stream before the interior
+kernel (k_wpanel_in); each such dispatch is attributed to the next interior
 dispatch, and one whole launch's traffic is reported as
 panel_launch<S> / panel4_launch<S> / panel6_launch<S>.  The last line,
 `traffic_bytes_per_launch: name=bytes, ...`, is what bench.py reads.
@@ -130,17 +130,18 @@ def launch_key(interior: str) -> str:
     m = re.match(r"k_wpanel_in<([^,]+), \d+, (\d+)>", interior)
     if not m:
         return ""
-    depth = {"2": "panel", "4": "panel4", "6": "panel6"}[m.group(2)]
+    depth = {"2": "panel", "4": "panel4", "6": "panel6", "8": "panel8"}[m.group(2)]
     return f"{depth}_launch<{m.group(1)}>"
 
 
 def groups(disp: List[Tuple[int, str, float]]) -> Dict[str, List[float]]:
     """Per launch key, the summed value of every whole launch: each perimeter
-    and ring dispatch belongs to the next interior dispatch."""
+    dispatch (the two-step sweep's k_wpanel_edge, the deep sweeps' frame
+    passes k_frame) belongs to the next interior dispatch."""
     out: Dict[str, List[float]] = defaultdict(list)
     pending = 0.0
     for _, name, v in disp:
-        if name.startswith(("k_wpanel_edge", "k_ring2")):
+        if name.startswith(("k_wpanel_edge", "k_frame")):
             pending += v
         elif name.startswith("k_wpanel_in"):
             out[launch_key(name)].append(pending + v)

@@ -1,7 +1,7 @@
-// Host-only check of the batched deep marches' geometry (batch_deep_grid,
-// batch_frame_waves and the perimeter waves' row and column ranges, restated
-// here from frame_wave / edge_march), for many channel shapes, chunk widths
-// and depths:
+// Host-only check of the deep marches' geometry (batch_deep_grid and
+// aligned_deep_grid, batch_frame_waves, and the perimeter waves' row and
+// column ranges, restated here from frame_wave / edge_march), for many
+// lattice shapes, chunk widths and depths, batched and single-lattice forms:
 //  * every pass's perimeter waves own disjoint nodes;
 //  * the last pass's frame and the interior waves' owned nodes tile the
 //    channel exactly (interior chunks may overlap each other: they write the
@@ -19,16 +19,21 @@
 using namespace lbm;
 namespace lbm { Layout make_layout(int lx, int ly); }
 
-static int check(int lx, int ly, int cx, int levels) {
+// strip_cx 0: the batched form (batch_deep_grid); else the single-lattice
+// form with strips in strip_cx-column chunks
+static int check(int lx, int ly, int cx, int levels, int strip_cx) {
     Layout L = make_layout(lx, ly);
     WaveGrid g; DeepFrame f;
-    if (!batch_deep_grid(L, cx, levels, &g, &f)) return -1;
+    if (strip_cx ? !aligned_deep_grid(L, cx, strip_cx, levels, &g, &f)
+                 : !batch_deep_grid(L, cx, levels, &g, &f))
+        return -1;
     const int H = levels - 1, nmid = levels / 2 - 1;
     auto idx = [&](int x, int y) { return (size_t)x * (ly + 2) + y; };
     std::vector<std::vector<int>> reg(nmid + 1, std::vector<int>((size_t)(lx + 2) * (ly + 2), 0));
     for (int j = 0; j <= nmid; ++j) {
         const int sr = f.sr[j];
         const int GL = sr + 2 <= 8 ? 8 : 16;
+        if (sr + 2 > 16) { printf("strip of %d rows beyond 16 lanes\n", sr); return 1; }
         const int nw = batch_frame_waves(f, j);
         for (int e = 0; e < nw; ++e) {
             int ee = e;
@@ -103,12 +108,18 @@ int main() {
     srand(1);
     std::vector<std::pair<int,int>> dims = {{700,250},{1030,140},{520,300},{130,33},{300,75},{64,16},{48,300},{17,5},{9,4},{128,32},{2048,512},{36,20},{40,20},{200,22}};
     for (int k = 0; k < 80; ++k) { int lx = 20 + rand() % 2100; dims.push_back({lx, std::max(3, lx / 4 + rand() % 7 - 3)}); }
-    for (auto [lx, ly] : dims)
+    for (auto [lx, ly] : dims) {
         for (int levels : {4, 6})
             for (int cx : {4, 6, 8, 12, 16, 24, 32, 48, 64}) {
-                int r = check(lx, ly, cx, levels);
+                int r = check(lx, ly, cx, levels, 0);
                 if (r < 0) ++skip; else if (r) { ++bad; printf("  bad: %dx%d cx %d levels %d\n", lx, ly, cx, levels); } else ++ok;
             }
+        for (int levels : {4, 6, 8})
+            for (int cx : {3, 7, 16, 24, 39, 63, 64}) {
+                int r = check(lx, ly, cx, levels, cx);
+                if (r < 0) ++skip; else if (r) { ++bad; printf("  bad: %dx%d cx %d levels %d (strips of cx)\n", lx, ly, cx, levels); } else ++ok;
+            }
+    }
     printf("ok %d bad %d skipped %d\n", ok, bad, skip);
     return bad != 0;
 }
