@@ -616,8 +616,10 @@ static void launch_in(const Problem &p, const WaveGrid &g, uint32_t dstep, const
     if (probe) {
         std::snprintf(probe->name, sizeof probe->name, "k_wpanel_in<%s, %d, %d>", type_name<S>(),
                       DEPTH, LEVELS);
-        // rows [r0, r1) of the interior chunks [c_lo, c_hi)
-        probe->nodes = (double)(g.r1 - g.r0) * (double)(g.c_hi - g.c_lo) * g.cx;
+        // rows [r0, r1) of the columns the interior chunks [c_lo, c_hi) cover
+        // (an aligned grid's leftmost chunk overlaps its neighbour)
+        const int cols = in_chunk_x1(g, g.c_hi - 1) - (in_chunk_x1(g, g.c_lo) - g.cx);
+        probe->nodes = (double)(g.r1 - g.r0) * (double)cols;
         (void)hipEventRecord(probe->t0, st);
     }
     k_wpanel_in<S, DEPTH, LEVELS><<<(g.nin + WPG - 1) / WPG, 64 * WPG, 0, st>>>(p, g, dstep, P, Q);

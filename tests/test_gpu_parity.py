@@ -150,6 +150,13 @@ def test_deep_sweeps_2000_steps(orc, variant):
     depth = {"tight": 1, "panel4": 4, "panel6": 6}[variant]
     launches = max(r["launches"] for r in recs if r["depth"] == depth)
     assert launches >= 1999 // depth - 1, recs
+    if depth > 1:
+        # the interior dispatch updates the rows and columns H + 2 .. ly - H - 1
+        # / lx - H - 1 once per level (H = depth - 1): each node counted once
+        # although the leftmost chunk overlaps its neighbour
+        h = depth - 1
+        rec = next(r for r in recs if r["depth"] == depth and "k_wpanel_in" in r["kernel"])
+        assert rec["nodes"] == (256 - 2 * h - 2) * (512 - 2 * h - 2), rec
 
 
 def test_resume_from_uploaded_state(orc):
