@@ -257,12 +257,15 @@ template <int N, typename F> __device__ __forceinline__ void static_for(F &&f) {
 // memory together as one contiguous run per population: eight waves.  The
 // six-step FP64 march (205 VGPRs with two columns in flight, two waves per
 // SIMD) takes four: an eight-wave workgroup needs two waves on every SIMD of
-// its CU, so no perimeter wave (242 VGPRs) fits beside it, while beside a
-// four-wave workgroup one does (16384 x 4096: 129.1-129.6 GLUPS against
+// its CU, so no perimeter wave (242 VGPRs in that measurement; the frame
+// kernel k_frame takes 184) fits beside it, while beside a four-wave
+// workgroup one does (16384 x 4096: 129.1-129.6 GLUPS against
 // 116.2-116.9, and 115.0 / 119.2 with two / one; the four-step sweep, more
 // memory-bound, loses with four: 95 against 113).  The six-step march with
 // one column in flight takes four as well (twelve would hold it to 168
-// VGPRs, where it spills).  A six-step variant at three waves per SIMD (the
+// VGPRs, where it spills); so does the eight-step FP64 march (252 VGPRs, one
+// column in flight: 3.69-3.74 ms per launch with four, 4.14 with eight).  A
+// six-step variant at three waves per SIMD (the
 // next level's c_x = +1 window in LDS, 164 VGPRs, one column in flight) ran
 // its interior kernel in 2.53 ms against 2.72, but then no perimeter wave
 // fits beside it until it drains, and the launch took 3.25 ms (2.70 with the
