@@ -82,3 +82,18 @@ def test_batch_launches_are_time_overlap_groups(tmp_path):
     assert sorted(got) == [("double", 2), ("double", 6)]
     assert [round(v, 6) for v in got[("double", 6)]] == [6.0, 6.0]
     assert [round(v, 6) for v in got[("double", 2)]] == [3.0]
+
+
+def test_panel_launch_traffic_sums_frame_passes():
+    """Whole-launch traffic of the panel sweeps: every perimeter dispatch
+    (k_wpanel_edge for two steps, k_frame for the deep sweeps' frame passes)
+    belongs to the next interior dispatch; other kernels start afresh."""
+    from lbm_amd import profile_report as pr
+    disp = [(1, "k_frame<double, 16>", 1.0), (2, "k_frame<double, 16>", 2.0),
+            (3, "k_frame<double, 8>", 3.0), (4, "k_wpanel_in<double, 2, 6>", 100.0),
+            (5, "k_bump", 0.5), (6, "k_wpanel_edge<float, 4>", 4.0),
+            (7, "k_wpanel_in<float, 2, 2>", 50.0), (8, "k_frame<double, 16>", 5.0),
+            (9, "k_collide<double, true, 2>", 7.0), (10, "k_wpanel_in<double, 1, 8>", 90.0)]
+    got = pr.groups(disp)
+    assert got == {"panel6_launch<double>": [106.0], "panel_launch<float>": [54.0],
+                   "panel8_launch<double>": [90.0]}
