@@ -114,18 +114,17 @@ class SolveServer:
     # size and format: SWEEPS lists (minimum cells, variant) in increasing
     # size.  Measured on MI355X, one solve alone, graphs of 50 steps, GLUPS
     # fused / tight_block / panel / panel4 / panel6:
-    #   FP64 512 x 128 9.8 / 9.8 / 4.7 / 4.8 / 4.8, 768 x 192 18.7 / 14.2 /
-    #   10.7 / 8.1 / 8.1, 1024 x 256 23.1 / 24.3 / 17.7 / 13.6 / 14.0,
-    #   1536 x 384 27.0 / 37.3 / 31.8 / 26.5 / 28.2, 2048 x 512 34.6 / 43.6 /
-    #   45.4 / 41.1 / 43.4, 3072 x 768 38.9 / 54.0 / 51.1 / 60.3 / 68.3,
-    #   4096 x 1024 38.7 / 59.7 / 56.9 / 75.6 / 85.6, 8192 x 2048 40.7 / 63.6 /
-    #   62.1 / 108.9 / 126.6;
-    #   FP32 512 x 128 8.1 / 5.4 / 4.3 / 4.2 / 4.2, 1024 x 256 26.4 / 16.6 /
-    #   16.1 / 10.8 / 11.4, 1536 x 384 33.1 / 31.9 / 34.7 / 24.6 / 25.2,
-    #   2048 x 512 46.8 / 42.1 / 51.9 / 36.9 / 35.6, 4096 x 1024 68.7 / 73.8 /
-    #   90.4 / 79.3 / 71.5, 8192 x 2048 77.5 / 88.2 / 120.6 / 122.8 / 118.5.
-    SWEEPS = {"f64": ((0, "combine"), (200_000, "tight_block"), (1_000_000, "panel"),
-                      (2_000_000, "panel6")),
+    #   FP64 512 x 128 9.8 / 9.8 / 4.7 / 4.7 / 4.7, 768 x 192 18.4 / 14.2 /
+    #   10.6 / 8.3 / 8.7, 1024 x 256 22.8 / 23.8 / 17.8 / 14.4 / 15.1,
+    #   1536 x 384 26.7 / 36.5 / 31.2 / 31.4 / 32.3, 2048 x 512 33.8 / 43.1 /
+    #   44.7 / 46.3 / 51.0, 3072 x 768 38.3 / 53.0 / 50.4 / 69.4 / 76.2,
+    #   4096 x 1024 38.1 / 57.7 / 52.9 / 82.6 / 95.8, 8192 x 2048 39.2 / 58.9 /
+    #   61.1 / 109.8 / 134.0;
+    #   FP32 512 x 128 8.1 / 5.4 / 4.2 / 4.2 / 4.2, 1024 x 256 26.8 / 16.4 /
+    #   15.9 / 12.8 / 12.8, 1536 x 384 33.7 / 32.0 / 34.4 / 27.7 / 28.7,
+    #   2048 x 512 47.0 / 42.1 / 51.9 / 42.3 / 40.0, 4096 x 1024 68.2 / 73.1 /
+    #   90.9 / 85.6 / 79.0, 8192 x 2048 77.3 / 88.1 / 110.7 / 133.1 / 126.6.
+    SWEEPS = {"f64": ((0, "combine"), (200_000, "tight_block"), (1_000_000, "panel6")),
               "f32": ((0, "combine"), (500_000, "panel"), (8_000_000, "panel4")),
               "f16": ((0, "combine"), (800_000, "panel"), (8_000_000, "panel4"))}
 
