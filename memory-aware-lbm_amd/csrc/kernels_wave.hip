@@ -598,6 +598,10 @@ static WaveGrid wave_grid_deep(const Layout &L, int cx, int storage, int levels)
 
 bool wave_grid_fits(const Layout &L) { return (int64_t)(3 + 16) * L.pitch * 8 < (int64_t)DROP; }
 
+int wpanel_deep_cx(const Layout &L, int cx, int storage, int levels) {
+    return wave_grid_deep(L, cx, storage, levels).cx;
+}
+
 bool wpanel_deep_possible(const Layout &L, int cx, int storage, int levels) {
     if (levels != 4 && levels != 6 && levels != 8) return false;
     if (!wave_grid_fits(L)) return false;
@@ -763,7 +767,7 @@ hipError_t launch_wpanel_deep(int storage, const Problem &p, int cx, int levels,
     const Layout &L = p.lay;
     // the chunk width of the thin-frame grid (its heuristics), the aligned grid
     // and frame over it, the strips in chunks of that width too
-    const int ccx = wave_grid_deep(L, cx, storage, levels).cx;
+    const int ccx = wpanel_deep_cx(L, cx, storage, levels);
     WaveGrid g;
     DeepFrame f;
     if (!aligned_deep_grid(L, ccx, ccx, levels, &g, &f)) return hipErrorInvalidValue;

@@ -124,6 +124,8 @@ hipError_t launch_wpanel(int storage, const Problem &p, const WaveGrid &g, const
 // interior for it (wpanel_deep_possible: an interior of at least 124 rows
 // and two chunks of the thin-frame grid).
 bool wpanel_deep_possible(const Layout &L, int cx, int storage, int levels);
+// the interior chunk width a deep launch takes (cx < 3: chosen for the lattice)
+int wpanel_deep_cx(const Layout &L, int cx, int storage, int levels);
 hipError_t launch_wpanel_deep(int storage, const Problem &p, int cx, int levels, const void *P,
                               void *const *mid, void *Q, hipStream_t st, hipStream_t side,
                               hipEvent_t fork, hipEvent_t join, KernelProbe *probe = nullptr);

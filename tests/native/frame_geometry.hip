@@ -9,7 +9,9 @@
 //  * pass j reads (its nodes and two more in every direction) only nodes that
 //    pass j - 1 wrote;
 //  * interior waves read only columns 0 .. lx + 1 and rows whose halo stays
-//    off the walls.
+//    off the walls;
+//  * every lattice the single-lattice deep sweeps accept gets such a grid at
+//    the chunk width its launch takes.
 // Test infrastructure: built and run by tests/test_frame_geometry.py; no GPU.
 #include <algorithm>
 #include <cstdio>
@@ -103,6 +105,20 @@ static int check(int lx, int ly, int cx, int levels, int strip_cx) {
     return 0;
 }
 
+// every lattice the single-lattice deep sweeps accept (wpanel_deep_possible)
+// gets an aligned grid at the chunk width the launch takes
+static int check_eligible(int lx, int ly, int storage, int levels) {
+    Layout L = make_layout(lx, ly);
+    if (!wpanel_deep_possible(L, 0, storage, levels)) return -1;
+    const int cx = wpanel_deep_cx(L, 0, storage, levels);
+    WaveGrid g; DeepFrame f;
+    if (!aligned_deep_grid(L, cx, cx, levels, &g, &f)) {
+        printf("eligible %dx%d storage %d levels %d but no aligned grid (cx %d)\n", lx, ly, storage, levels, cx);
+        return 1;
+    }
+    return check(lx, ly, cx, levels, cx);
+}
+
 int main() {
     int bad = 0, ok = 0, skip = 0;
     srand(1);
@@ -120,6 +136,14 @@ int main() {
                 if (r < 0) ++skip; else if (r) { ++bad; printf("  bad: %dx%d cx %d levels %d (strips of cx)\n", lx, ly, cx, levels); } else ++ok;
             }
     }
+    // the deep sweeps' eligibility on lattice shapes around its thresholds
+    for (int ly : {120, 130, 135, 136, 137, 140, 144, 150, 200, 256, 333, 700})
+        for (int lx : {40, 64, 100, 128, 200, 300, 512, 700, 1500})
+            for (int storage : {0, 1, 2})
+                for (int levels : {4, 6, 8}) {
+                    int r = check_eligible(lx, ly, storage, levels);
+                    if (r < 0) ++skip; else if (r) { ++bad; printf("  bad: eligible %dx%d\n", lx, ly); } else ++ok;
+                }
     printf("ok %d bad %d skipped %d\n", ok, bad, skip);
     return bad != 0;
 }
