@@ -586,11 +586,18 @@ WaveGrid wave_grid(const Layout &L, int cx) {
 // flight better than two (odd widths: 39 / 47 / 55 / 63 columns 106.7 /
 // 107.2 / 107.5 / 107.7 GLUPS against 105.7 / 105.6 / 104.9 / 106.0 at 40 /
 // 48 / 56 / 64) and takes 63.  Six steps (ten halo columns per chunk) take
-// 64: FP64 101 / 101 / 106 at 47 / 55 / 64.
+// 64 (FP64 101 / 101 / 106 at 47 / 55 / 64), and FP64 96 once the frame's
+// strips run in FRAME_CX-column chunks: the six-step launch at 64 / 96 / 128
+// / 192 columns 149.4-150.0 / 150.7-151.8 / 150.4-152.1 / 146.2-147.8 GLUPS,
+// the 6 + 6 + 8 plan 147.1-149.5 / 150.0-151.3 / 148.9-149.6 / 147.6-148.3.
 static WaveGrid wave_grid_deep(const Layout &L, int cx, int storage, int levels) {
     const int64_t bands = (L.ly + WAVE_ROWS - 1) / WAVE_ROWS;
     const bool largest = bands * L.lx >= 128 * 6144;
-    const int cmax = !largest ? 24 : levels >= 6 ? 64 : storage != 2 ? 40 : 24;
+    const int cmax = !largest                      ? 24
+                     : levels >= 6 && storage == 0 ? 96
+                     : levels >= 6                 ? 64
+                     : storage != 2                ? 40
+                                                   : 24;
     if (cx < 3 && largest && levels == 4 && storage == 0) cx = 63;
     // at least two bands of interior rows
     return make_thin(L, wave_grid_max(L, cx, cmax, 6144), levels - 1, 2 * WAVE_ROWS);
@@ -679,8 +686,12 @@ hipError_t launch_wpanel(int storage, const Problem &p, const WaveGrid &g, const
 // lanes per strip chunk of a frame pass with sr strip rows (and two halo rows)
 constexpr int frame_gl(int sr) { return sr + 2 <= 8 ? 8 : 16; }
 
-// columns per strip chunk of a batched deep march's frame
+// columns per strip chunk of a deep march's frame: batched channels, and
+// single lattices whose interior chunks are wider (the strips' two-step
+// march runs a chunk's columns in sequence, so chunks as wide as the
+// interior's 96 made the three frame passes outlast the interior kernel)
 constexpr int FRAME_CX = 16;
+int deep_strip_cx(int cx) { return cx > FRAME_CX ? FRAME_CX : cx; }
 
 // strip rows of pass j of a LEVELS-step frame (0 past the last pass)
 constexpr int frame_rows(int levels, int j) {
@@ -766,11 +777,11 @@ hipError_t launch_wpanel_deep(int storage, const Problem &p, int cx, int levels,
     if (!wpanel_deep_possible(p.lay, cx, storage, levels)) return hipErrorInvalidValue;
     const Layout &L = p.lay;
     // the chunk width of the thin-frame grid (its heuristics), the aligned grid
-    // and frame over it, the strips in chunks of that width too
+    // and frame over it, the strips in chunks of at most FRAME_CX columns
     const int ccx = wpanel_deep_cx(L, cx, storage, levels);
     WaveGrid g;
     DeepFrame f;
-    if (!aligned_deep_grid(L, ccx, ccx, levels, &g, &f)) return hipErrorInvalidValue;
+    if (!aligned_deep_grid(L, ccx, deep_strip_cx(ccx), levels, &g, &f)) return hipErrorInvalidValue;
     const int nmid = levels / 2 - 1;
     // two columns of loads in flight when the chunk width allows, except for
     // the eight-step FP64 march (two columns in flight need 265 registers,

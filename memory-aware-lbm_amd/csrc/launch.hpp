@@ -190,6 +190,8 @@ bool batch_deep_grid(const Layout &L, int cx, int levels, WaveGrid *g, DeepFrame
 bool aligned_deep_grid(const Layout &L, int cx, int strip_cx, int levels, WaveGrid *g,
                        DeepFrame *f);
 int batch_frame_waves(const DeepFrame &f, int j);
+// the strip chunk width of a single lattice's deep march in cx-column chunks
+int deep_strip_cx(int cx);
 int batch_deep_wpg(int storage, int levels);
 hipError_t launch_batch_wpanel_deep(int storage, int levels, const BatchTables &deep,
                                     const int2 *const *frame_items, const int *nframe,

@@ -112,11 +112,11 @@ static int check_eligible(int lx, int ly, int storage, int levels) {
     if (!wpanel_deep_possible(L, 0, storage, levels)) return -1;
     const int cx = wpanel_deep_cx(L, 0, storage, levels);
     WaveGrid g; DeepFrame f;
-    if (!aligned_deep_grid(L, cx, cx, levels, &g, &f)) {
+    if (!aligned_deep_grid(L, cx, deep_strip_cx(cx), levels, &g, &f)) {
         printf("eligible %dx%d storage %d levels %d but no aligned grid (cx %d)\n", lx, ly, storage, levels, cx);
         return 1;
     }
-    return check(lx, ly, cx, levels, cx);
+    return check(lx, ly, cx, levels, deep_strip_cx(cx));
 }
 
 int main() {
@@ -144,6 +144,16 @@ int main() {
                     int r = check_eligible(lx, ly, storage, levels);
                     if (r < 0) ++skip; else if (r) { ++bad; printf("  bad: eligible %dx%d\n", lx, ly); } else ++ok;
                 }
+    // the largest lattice at the chunk widths its deep launches take or may be
+    // given, strips in chunks of at most FRAME_CX columns
+    for (int levels : {4, 6, 8}) {
+        int r = check_eligible(16384, 4096, 0, levels);
+        if (r < 0) ++skip; else if (r) { ++bad; printf("  bad: eligible 16384x4096 levels %d\n", levels); } else ++ok;
+        for (int cx : {40, 63, 64, 96, 128}) {
+            r = check(16384, 4096, cx, levels, deep_strip_cx(cx));
+            if (r < 0) ++skip; else if (r) { ++bad; printf("  bad: 16384x4096 cx %d levels %d\n", cx, levels); } else ++ok;
+        }
+    }
     printf("ok %d bad %d skipped %d\n", ok, bad, skip);
     return bad != 0;
 }
