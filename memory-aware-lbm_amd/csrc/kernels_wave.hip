@@ -687,11 +687,15 @@ hipError_t launch_wpanel(int storage, const Problem &p, const WaveGrid &g, const
 constexpr int frame_gl(int sr) { return sr + 2 <= 8 ? 8 : 16; }
 
 // columns per strip chunk of a deep march's frame: batched channels, and
-// single lattices whose interior chunks are wider (the strips' two-step
-// march runs a chunk's columns in sequence, so chunks as wide as the
-// interior's 96 made the three frame passes outlast the interior kernel)
+// single lattices whose interior chunks are wider than 64 columns (the
+// strips' two-step march runs a chunk's columns in sequence, so strips as
+// wide as 96-column interior chunks made the three frame passes outlast the
+// interior kernel).  Up to 64 columns the strips take the interior's width:
+// 16-column strips cost the FP64 four-step sweep at 63 columns 2.5% (four
+// times the perimeter waves beside its three-wave-per-SIMD interior: 122.4 /
+// 123.2 -> 119.6 / 120.2 GLUPS), FP32 at 40 columns 0.7%.
 constexpr int FRAME_CX = 16;
-int deep_strip_cx(int cx) { return cx > FRAME_CX ? FRAME_CX : cx; }
+int deep_strip_cx(int cx) { return cx > 64 ? FRAME_CX : cx; }
 
 // strip rows of pass j of a LEVELS-step frame (0 past the last pass)
 constexpr int frame_rows(int levels, int j) {
@@ -777,7 +781,7 @@ hipError_t launch_wpanel_deep(int storage, const Problem &p, int cx, int levels,
     if (!wpanel_deep_possible(p.lay, cx, storage, levels)) return hipErrorInvalidValue;
     const Layout &L = p.lay;
     // the chunk width of the thin-frame grid (its heuristics), the aligned grid
-    // and frame over it, the strips in chunks of at most FRAME_CX columns
+    // and frame over it, the strips in chunks of deep_strip_cx columns
     const int ccx = wpanel_deep_cx(L, cx, storage, levels);
     WaveGrid g;
     DeepFrame f;

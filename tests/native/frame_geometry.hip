@@ -145,7 +145,7 @@ int main() {
                     if (r < 0) ++skip; else if (r) { ++bad; printf("  bad: eligible %dx%d\n", lx, ly); } else ++ok;
                 }
     // the largest lattice at the chunk widths its deep launches take or may be
-    // given, strips in chunks of at most FRAME_CX columns
+    // given, strips as wide as the launch makes them
     for (int levels : {4, 6, 8}) {
         int r = check_eligible(16384, 4096, 0, levels);
         if (r < 0) ++skip; else if (r) { ++bad; printf("  bad: eligible 16384x4096 levels %d\n", levels); } else ++ok;
