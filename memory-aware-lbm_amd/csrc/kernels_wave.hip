@@ -589,7 +589,9 @@ WaveGrid wave_grid(const Layout &L, int cx) {
 // 64 (FP64 101 / 101 / 106 at 47 / 55 / 64), and FP64 96 once the frame's
 // strips run in FRAME_CX-column chunks: the six-step launch at 64 / 96 / 128
 // / 192 columns 149.4-150.0 / 150.7-151.8 / 150.4-152.1 / 146.2-147.8 GLUPS,
-// the 6 + 6 + 8 plan 147.1-149.5 / 150.0-151.3 / 148.9-149.6 / 147.6-148.3.
+// the 6 + 6 + 8 plan 147.1-149.5 / 150.0-151.3 / 148.9-149.6 / 147.6-148.3;
+// the eight-step FP64 launch too: 3.73-3.79 ms at 64 columns, 3.60-3.78 at
+// 96, 3.73-3.80 at 128, 3.77-3.80 at 160 (six-step launches at 96).
 static WaveGrid wave_grid_deep(const Layout &L, int cx, int storage, int levels) {
     const int64_t bands = (L.ly + WAVE_ROWS - 1) / WAVE_ROWS;
     const bool largest = bands * L.lx >= 128 * 6144;
