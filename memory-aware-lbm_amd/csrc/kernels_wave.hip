@@ -580,7 +580,8 @@ WaveGrid wave_grid(const Layout &L, int cx) {
 // 85 / 86, 16384 x 4096 - / 91 / 97 / 100.  Only the largest lattices gain
 // from chunks wider still (fewer halo columns re-read): 16384 x 4096 at 24 /
 // 40 / 64 columns runs FP64 104 / 106 / 107, FP32 128 / 134 / 132, but FP16
-// 217 / 204 / 187, and 8192 x 2048 loses at 40 in every format (FP32 105 at
+// 217 / 204 / 187 (frame strips as wide as the interior chunks, as below),
+// and 8192 x 2048 loses at 40 in every format (FP32 105 at
 // 24, 91 at 40); four steps take 40 columns in FP64 and FP32 where 24 would
 // leave more than 128 x 6144 waves.  There, FP64 runs one column of loads in
 // flight better than two (odd widths: 39 / 47 / 55 / 63 columns 106.7 /
@@ -591,15 +592,18 @@ WaveGrid wave_grid(const Layout &L, int cx) {
 // / 192 columns 149.4-150.0 / 150.7-151.8 / 150.4-152.1 / 146.2-147.8 GLUPS,
 // the 6 + 6 + 8 plan 147.1-149.5 / 150.0-151.3 / 148.9-149.6 / 147.6-148.3;
 // the eight-step FP64 launch too: 3.73-3.79 ms at 64 columns, 3.60-3.78 at
-// 96, 3.73-3.80 at 128, 3.77-3.80 at 160 (six-step launches at 96).
+// 96, 3.73-3.80 at 128, 3.77-3.80 at 160 (six-step launches at 96).  With
+// FRAME_CX-column strips the reduced formats' four-step sweeps take 96 as
+// well: FP16 231-235 GLUPS at 24 columns, 238-241 at 40, 246-250 at 96;
+// FP32 140 at 40, 143-144 at 96 (FP64 at 95: 123.3-123.5, as at 63).
 static WaveGrid wave_grid_deep(const Layout &L, int cx, int storage, int levels) {
     const int64_t bands = (L.ly + WAVE_ROWS - 1) / WAVE_ROWS;
     const bool largest = bands * L.lx >= 128 * 6144;
     const int cmax = !largest                      ? 24
                      : levels >= 6 && storage == 0 ? 96
                      : levels >= 6                 ? 64
-                     : storage != 2                ? 40
-                                                   : 24;
+                     : storage != 0                ? 96
+                                                   : 40;
     if (cx < 3 && largest && levels == 4 && storage == 0) cx = 63;
     // at least two bands of interior rows
     return make_thin(L, wave_grid_max(L, cx, cmax, 6144), levels - 1, 2 * WAVE_ROWS);

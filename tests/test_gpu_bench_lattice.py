@@ -10,10 +10,12 @@ launch and a single step; panel4: three four-step launches, then two, then a
 two-step launch and a single step).  Each result is compared with the
 oracle's (the reference's origin order, lb.c:172-341, restated in C) on all
 67,108,864 interior nodes, bit for bit: the sweeps this lattice exercises
-and the small-lattice tests do not are its ~19,000 interior waves per
+and the small-lattice tests do not are its 13,000-67,000 interior waves per
 launch, the XCD renumbering over that many workgroups, byte offsets above
-4 GiB into the population planes, and the deep sweeps' frame and rings
-along 66 bands and 256 chunks (lb.c:353-437, 936-1069, 1402-1538)."""
+4 GiB into the population planes, and the deep sweeps' frames along 66-76
+bands and 171-260 chunks of 63-96 columns (lb.c:353-437, 936-1069,
+1402-1538).  The FP32 and FP16 forms of the deep sweeps, at the chunk widths
+they take here, are compared with their one-step sweep the same way."""
 import numpy as np
 import pytest
 
@@ -61,3 +63,26 @@ def test_bench_lattice_matches_oracle(reference, variant):
         pytest.fail(f"{variant}: differs from the oracle on 16384x4096 (max-abs {d:g}, "
                     f"lattice equal {ok_l}, tmpLattice equal {ok_t})")
     assert np.array_equal(po[2:LY], pref[2:LY])
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_bench_lattice_reduced_deep_sweeps_equal_single_steps(storage):
+    """FP32 / FP16 storage on the same lattice and step calls: panel4 (96-column
+    chunks here, frame strips of 16 columns) and panel6 (64-column chunks)
+    give the bits of the one-step sweep (FP32: the same stochastic rounding
+    per collision index)."""
+    dev = L.Channel.geometry(LX, LY, "B")
+
+    def run(variant):
+        with L.Lattice(dev, storage=storage) as lat:
+            lat.init_equilibrium()
+            for n in CALLS:
+                lat.step(n, variant)
+            return lat.download()
+
+    ref = run("combine")
+    for variant in ("panel4", "panel6"):
+        got = run(variant)
+        for name, a, b in zip(("lattice", "tmpLattice", "outlet rho"), got, ref):
+            assert np.array_equal(a, b), f"{storage} {variant}: {name} differs from combine"
+        del got
