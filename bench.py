@@ -19,8 +19,11 @@ deep sweep are as many six-step launches as fit, then the launches of the
 remainder (a remainder of 2 or 3 as one eight-step launch in place of the
 last six-step one); `steps_by_depth` says how many steps ran at each depth.
 The other variants are timed on the same lattice ("variants"), with the FP32
-and FP16 storage forms ("storage_forms").  The populations are initialised on
-the device, so everything is resident in HBM when the timed region starts.
+and FP16 storage forms ("storage_forms"): their `glups` over a timed region
+without launch events, their kernel times (and `glups_with_launch_events`)
+from a second pass of as many steps with them.  The populations are
+initialised on the device, so everything is resident in HBM when the timed
+region starts.
 
 roofline: the dominant kernel is the one that carries the sweep's lattice
 updates (for panel6 the interior march k_wpanel_in<double, 2, 6>, whose
@@ -136,7 +139,15 @@ def dominant(recs):
     return max(recs, key=lambda r: (r["launches"] * r["avg_ms"], r["depth"]))
 
 
-def time_variant(lbm, ch, variant, steps, warmup, storage="f64", settle=0):
+def time_variant(lbm, ch, variant, steps, warmup, storage="f64", settle=0, probed=True):
+    """Throughput of `steps` steps of a sweep and the launch times of its
+    dominant kernel.  probed: the launch times come from HIP events recorded
+    around every launch inside the timed region itself (the headline);
+    otherwise the timed region runs without them and a second pass of as many
+    steps records them.  The events change how a deep sweep's interior kernel
+    overlaps its frame kernels on the other stream (FP32 four-step sweep on
+    16384x4096: 149.6 GLUPS without them, 134.5 with), so the sweeps other
+    than the headline report the throughput measured without them."""
     with lbm.Lattice(ch, storage=storage) as lat:
         if settle:
             # run the sweep on this lattice first so the GPU's clocks, its
@@ -147,10 +158,14 @@ def time_variant(lbm, ch, variant, steps, warmup, storage="f64", settle=0):
         lat.init_equilibrium()
         lat.step(max(warmup, 1), variant)   # leaves the state in the variant's convention
         lat.sync()
+        if not probed:
+            ms = lat.time_steps(steps, variant)
         lat.kernel_timing(True)
-        ms = lat.time_steps(steps, variant)
+        ms_probed = lat.time_steps(steps, variant)
         recs = lat.kernel_times()
         lat.kernel_timing(False)
+        if probed:
+            ms = ms_probed
     cells = ch.lx * ch.ly
     dom = dominant(recs)
     bpu = BYTES_PER_UPDATE[storage]
@@ -159,6 +174,7 @@ def time_variant(lbm, ch, variant, steps, warmup, storage="f64", settle=0):
     for r in recs:
         by_depth[str(r["depth"])] = by_depth.get(str(r["depth"]), 0) + r["depth"] * r["launches"]
     return dict(ms=ms, ms_per_step=ms / steps, glups=cells * steps / (ms * 1e-3) / 1e9,
+                glups_probed=cells * steps / (ms_probed * 1e-3) / 1e9,
                 kernel=dom["kernel"], depth=dom["depth"], kernel_avg_ms=dom["avg_ms"],
                 kernel_nodes=dom["nodes"], achieved=achieved, steps_by_depth=by_depth,
                 launches=[{"kernel": r["kernel"], "depth": r["depth"], "launches": r["launches"],
@@ -183,6 +199,7 @@ def measured_traffic(key):
 
 def summary(r, storage):
     out = {"glups": round(r["glups"], 4), "ms_per_step": round(r["ms_per_step"], 5),
+           "glups_with_launch_events": round(r["glups_probed"], 4),
            "kernel": r["kernel"], "steps_per_launch": r["depth"],
            "kernel_avg_ms": round(r["kernel_avg_ms"], 5),
            "nodes_per_launch": int(r["kernel_nodes"]),
@@ -380,12 +397,14 @@ def main():
     head = time_variant(lbm, ch, args.variant, args.steps, args.warmup, settle=args.settle)
     others = {}
     for v in [v for v in args.variants.split(",") if v and v != args.variant]:
-        others[v] = summary(time_variant(lbm, ch, v, args.other_steps, min(args.warmup, 20)), "f64")
+        others[v] = summary(time_variant(lbm, ch, v, args.other_steps, min(args.warmup, 20),
+                                         probed=False), "f64")
     storage = {}
     for sname in [v for v in args.storages.split(",") if v]:
         storage[sname] = {"bytes_per_update": BYTES_PER_UPDATE[sname]}
         for v in ("combine", "tight_block", "panel", "panel4"):
-            r = time_variant(lbm, ch, v, args.other_steps, min(args.warmup, 20), sname)
+            r = time_variant(lbm, ch, v, args.other_steps, min(args.warmup, 20), sname,
+                             probed=False)
             storage[sname][v] = summary(r, sname)
             f64 = head if v == args.variant else None
             base = (f64 or {}).get("glups") or others.get(v, {}).get("glups")
