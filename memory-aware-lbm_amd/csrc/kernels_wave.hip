@@ -595,15 +595,13 @@ WaveGrid wave_grid(const Layout &L, int cx) {
 // 96, 3.73-3.80 at 128, 3.77-3.80 at 160 (six-step launches at 96).  With
 // FRAME_CX-column strips the reduced formats' four-step sweeps take 96 as
 // well: FP16 231-235 GLUPS at 24 columns, 238-241 at 40, 246-250 at 96;
-// FP32 140 at 40, 143-144 at 96 (FP64 at 95: 123.3-123.5, as at 63).
+// FP32 140 at 40, 143-144 at 96 (FP64 at 95: 123.3-123.5, as at 63); so do
+// their six-step sweeps (FP32 147.0-147.5 GLUPS at 64, 147.6-149.0 at 96;
+// FP16 233.3-233.4 and 235.4-236.1).
 static WaveGrid wave_grid_deep(const Layout &L, int cx, int storage, int levels) {
     const int64_t bands = (L.ly + WAVE_ROWS - 1) / WAVE_ROWS;
     const bool largest = bands * L.lx >= 128 * 6144;
-    const int cmax = !largest                      ? 24
-                     : levels >= 6 && storage == 0 ? 96
-                     : levels >= 6                 ? 64
-                     : storage != 0                ? 96
-                                                   : 40;
+    const int cmax = !largest ? 24 : levels >= 6 || storage != 0 ? 96 : 40;
     if (cx < 3 && largest && levels == 4 && storage == 0) cx = 63;
     // at least two bands of interior rows
     return make_thin(L, wave_grid_max(L, cx, cmax, 6144), levels - 1, 2 * WAVE_ROWS);

@@ -67,9 +67,9 @@ def test_bench_lattice_matches_oracle(reference, variant):
 
 @pytest.mark.parametrize("storage", ["f32", "f16"])
 def test_bench_lattice_reduced_deep_sweeps_equal_single_steps(storage):
-    """FP32 / FP16 storage on the same lattice and step calls: panel4 (96-column
-    chunks here, frame strips of 16 columns) and panel6 (64-column chunks)
-    give the bits of the one-step sweep (FP32: the same stochastic rounding
+    """FP32 / FP16 storage on the same lattice and step calls: panel4 and
+    panel6 (96-column chunks here, frame strips of 16 columns; panel6's
+    remainder as an eight-step launch) give the bits of the one-step sweep (FP32: the same stochastic rounding
     per collision index)."""
     dev = L.Channel.geometry(LX, LY, "B")
 
